@@ -1,0 +1,129 @@
+"""ctypes binding of the plain-C restatement (oracle/_build/liboracle.so).
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it."""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+class _Mech(ctypes.Structure):
+    _fields_ = [("nvar", ctypes.c_int), ("nfix", ctypes.c_int), ("nreact", ctypes.c_int),
+                ("lu_nonzero", ctypes.c_int),
+                ("rx_ptr", _I), ("rx_spc", _I), ("rx_mult", _I),
+                ("st_ptr", _I), ("st_spc", _I), ("st_coef", _D),
+                ("lu_irow", _I), ("lu_icol", _I), ("lu_crow", _I), ("lu_diag", _I),
+                ("lu_map", _I)]
+
+
+def lib_path():
+    return _HERE / "_build" / "liboracle.so"
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Oracle:
+    """Per-cell arrays in the oracle API are [ncell][n] (species fastest);
+    the helpers below take and return the C-ABI layout [n][ncell]."""
+
+    def __init__(self, mech):
+        self.m = mech
+        self.lib = ctypes.CDLL(str(lib_path()))
+        rx_ptr, rx_spc, rx_mult, st_ptr, st_spc, st_coef = [0], [], [], [0], [], []
+        for rx in mech.reactions:
+            for s, k in rx["reactants"]:
+                rx_spc.append(s)
+                rx_mult.append(k)
+            rx_ptr.append(len(rx_spc))
+            for s, c in rx["stoich"]:
+                st_spc.append(s)
+                st_coef.append(c)
+            st_ptr.append(len(st_spc))
+        nv = mech.nvar
+        lu_map = -np.ones(nv * nv, dtype=np.int32)
+        if mech.lu_nonzero > 1:
+            lu_map[mech.lu_irow * nv + mech.lu_icol] = np.arange(mech.lu_nonzero, dtype=np.int32)
+        self._keep = [_i32(rx_ptr), _i32(rx_spc), _i32(rx_mult), _i32(st_ptr), _i32(st_spc),
+                      np.ascontiguousarray(st_coef, dtype=np.float64),
+                      _i32(mech.lu_irow), _i32(mech.lu_icol), _i32(mech.lu_crow),
+                      _i32(mech.lu_diag), lu_map]
+        k = self._keep
+        P = lambda a: a.ctypes.data_as(_D if a.dtype == np.float64 else _I)
+        self.ms = _Mech(nv, mech.nfix, mech.nreact, mech.lu_nonzero, *[P(a) for a in k])
+        L = self.lib
+        mp = ctypes.POINTER(_Mech)
+        L.orc_fun.argtypes = [mp, _D, _D, _D, _D]
+        L.orc_jac.argtypes = [mp, _D, _D, _D, _D]
+        L.orc_decomp.argtypes = [mp, _D]
+        L.orc_decomp.restype = ctypes.c_int
+        L.orc_solve.argtypes = [mp, _D, _D]
+        L.orc_integrate.argtypes = [mp, ctypes.c_int64, _D, _D, _D, ctypes.c_double,
+                                    ctypes.c_double, _D, _D, _I, _D, _I, _D, _I, ctypes.c_int]
+
+    def _p(self):
+        return ctypes.byref(self.ms)
+
+    def fun(self, conc, rct):
+        nv = self.m.nvar
+        out = np.zeros((nv, conc.shape[1]))
+        for c in range(conc.shape[1]):
+            V = np.ascontiguousarray(conc[:nv, c]); F = np.ascontiguousarray(conc[nv:, c])
+            R = np.ascontiguousarray(rct[:, c]); o = np.zeros(nv)
+            self.lib.orc_fun(self._p(), V.ctypes.data_as(_D), F.ctypes.data_as(_D),
+                             R.ctypes.data_as(_D), o.ctypes.data_as(_D))
+            out[:, c] = o
+        return out
+
+    def jac(self, conc, rct):
+        nv = self.m.nvar
+        out = np.zeros((self.m.lu_nonzero, conc.shape[1]))
+        for c in range(conc.shape[1]):
+            V = np.ascontiguousarray(conc[:nv, c]); F = np.ascontiguousarray(conc[nv:, c])
+            R = np.ascontiguousarray(rct[:, c]); o = np.zeros(self.m.lu_nonzero)
+            self.lib.orc_jac(self._p(), V.ctypes.data_as(_D), F.ctypes.data_as(_D),
+                             R.ctypes.data_as(_D), o.ctypes.data_as(_D))
+            out[:, c] = o
+        return out
+
+    def decomp(self, jvs):
+        out = np.array(jvs, dtype=np.float64, copy=True)
+        ier = np.zeros(jvs.shape[1], dtype=np.int32)
+        for c in range(jvs.shape[1]):
+            v = np.ascontiguousarray(out[:, c])
+            ier[c] = self.lib.orc_decomp(self._p(), v.ctypes.data_as(_D))
+            out[:, c] = v
+        return out, ier
+
+    def solve(self, lu, x):
+        out = np.array(x, dtype=np.float64, copy=True)
+        for c in range(x.shape[1]):
+            v = np.ascontiguousarray(out[:, c]); L = np.ascontiguousarray(lu[:, c])
+            self.lib.orc_solve(self._p(), L.ctypes.data_as(_D), v.ctypes.data_as(_D))
+            out[:, c] = v
+        return out
+
+    def integrate(self, conc, rct, hstart, tin, tout, icntrl, rcntrl, atol=None, rtol=None,
+                  nthreads=1):
+        n = conc.shape[1]
+        c = np.ascontiguousarray(conc.T, dtype=np.float64).copy()
+        r = np.ascontiguousarray(rct.T, dtype=np.float64)
+        ist = np.zeros((n, 20), dtype=np.int32)
+        rst = np.zeros((n, 20))
+        ierr = np.zeros(n, dtype=np.int32)
+        atol = np.ascontiguousarray(self.m.atol() if atol is None else atol)
+        rtol = np.ascontiguousarray(self.m.rtol() if rtol is None else rtol)
+        ic = _i32(icntrl)
+        rc = np.ascontiguousarray(rcntrl, dtype=np.float64)
+        hs = np.ascontiguousarray(hstart, dtype=np.float64)
+        self.lib.orc_integrate(self._p(), n, c.ctypes.data_as(_D), r.ctypes.data_as(_D),
+                               hs.ctypes.data_as(_D), tin, tout, atol.ctypes.data_as(_D),
+                               rtol.ctypes.data_as(_D), ic.ctypes.data_as(_I),
+                               rc.ctypes.data_as(_D), ist.ctypes.data_as(_I),
+                               rst.ctypes.data_as(_D), ierr.ctypes.data_as(_I), int(nthreads))
+        return np.ascontiguousarray(c.T), np.ascontiguousarray(ist.T), np.ascontiguousarray(rst.T), ierr
