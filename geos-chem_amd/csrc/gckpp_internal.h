@@ -1,0 +1,75 @@
+// Internal declarations shared by the HIP translation units of libgckpp_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gckpp {
+
+enum { MECH_FULLCHEM = 0, MECH_HG = 1, MECH_CARBON = 2 };
+
+struct MechInfo {
+  int nspec, nvar, nfix, nreact, lu_nz, npart;
+};
+
+// Integrator settings shared by all cells, resolved on the host from
+// ICNTRL/RCNTRL exactly as Rosenbrock() does (gckpp_Integrator.F90:165-532).
+struct RosParams {
+  double Tstart, Tend, Hmin, Hmax, Hstart_default, FacMin, FacMax, FacRej, FacSafe;
+  double roundoff, DeltaMin, direction;
+  int max_steps, vectol, autonomous, clip_negative;
+};
+
+struct IntegrateArgs {
+  int64_t ncell;
+  double *conc;          // [NSPEC][cstride], Y = rows 0..NVAR-1
+  int64_t cstride;
+  const double *rconst;  // [NREACT][rstride]
+  int64_t rstride;
+  const double *hstart;  // [ncell] or null
+  const double *atol, *rtol;  // [NVAR] device
+  double *work;          // per-cell workspace [wsize][wstride]
+  int64_t wstride;
+  int32_t *istatus;      // [20][sstride] or null
+  double *rstatus;       // [20][sstride] or null
+  int32_t *ierr;         // [ncell] or null
+  int64_t sstride;
+  const int32_t *only_failed;  // if set, solve only cells with only_failed[cell] < 0
+  const double *restore_from;  // if set, solved cells first copy conc from here
+  RosParams prm;
+};
+
+struct RatesArgs {
+  int64_t ncell;
+  const double *met;     // [NMET][ncell]
+  const double *photol;  // [NPHOTOL][ncell]
+  const double *het;     // [NHET][het_stride]
+  int64_t het_stride;    // 0 or ncell
+  const double *conc;    // [NSPEC][ncell]
+  double *rconst;        // [NREACT][ncell]
+  const double *consts;  // [3][NSPEC]: MW, HENRY_K0, HENRY_CR
+};
+
+struct CompArgs {
+  int64_t ncell;
+  const double *in;
+  int64_t istride;
+  const double *rconst;
+  int64_t rstride;
+  double *out;
+  int64_t ostride;
+  double *work;
+  int64_t wstride;
+  int32_t *ier;
+};
+
+MechInfo mech_info(int mech);
+// doubles of per-cell workspace the integrator needs
+inline int64_t integrate_work_per_cell(const MechInfo &m) {
+  int64_t scr = m.nreact > m.npart ? m.nreact : m.npart;
+  return (int64_t)m.lu_nz + 6 * (int64_t)m.nvar + scr;
+}
+void launch_integrate_mech(int mech, const IntegrateArgs &a, hipStream_t s);
+void launch_comp_mech(int mech, int which, const CompArgs &a, hipStream_t s);
+
+}  // namespace gckpp

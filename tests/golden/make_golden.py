@@ -1,0 +1,54 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference
+(oracle/_ref, built by `make -C oracle ref` from the reference checkout).
+
+Fixture <mech>_box.npz: a few box cells of the synthetic box workload (half
+dark, half lit) with the reference's outputs on them:
+  idx, met, photol, het, conc          inputs ([field][ncell])
+  rconst                               Update_RCONST (inlined literals filled)
+  vdot                                 Fun_SPLIT(conc, rconst)
+  jvs                                  Jac_SP
+  ghimj, lu, ier                       1/(h*gamma)-J with h=10 s, KppDecomp
+  x, xs                                a random right-hand side, KppSolve
+  int_conc, int_istatus, int_rstatus, int_ierr
+                                       Integrate(0,1200) from Hstart=1e-5 s
+  step_conc, step_hnew, step_ierr      Do_FullChem step (rates+integrate+retry+clip)
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "geos-chem_amd"))
+
+from gckpp_amd import mechanism, synth  # noqa: E402
+from oracle import ref  # noqa: E402
+
+
+def make(mech_name, cells):
+    m = mechanism.load(mech_name)
+    R = ref.RefKPP(m)
+    w = synth.box_cells(m, 4096, 0).subset(cells)
+    ic, rc = ref.fullchem_icntrl()
+    rct = R.rates(w)
+    vdot = R.fun(w.conc, rct)
+    jvs = R.jac(w.conc, rct)
+    ghimj = -jvs.copy()
+    ghimj[m.lu_diag[: m.nvar]] += 1.0 / (10.0 * 0.5)
+    lu, ier = R.decomp(ghimj)
+    x = np.random.default_rng(7).standard_normal((m.nvar, len(cells)))
+    xs = R.solve(lu, x)
+    c1, i1, r1, e1 = R.integrate(w.conc, rct, w.hstart, 0.0, 1200.0, ic, rc)
+    c2, i2, h2, e2 = R.fullchem_step(w, ic, rc, nthreads=1)
+    out = ROOT / "tests" / "golden" / ("%s_box.npz" % mech_name)
+    np.savez_compressed(out, idx=np.asarray(cells), met=w.met, photol=w.photol, het=w.het,
+                        conc=w.conc, rconst=rct, vdot=vdot, jvs=jvs, ghimj=ghimj, lu=lu, ier=ier,
+                        x=x, xs=xs, int_conc=c1, int_istatus=i1, int_rstatus=r1, int_ierr=e1,
+                        step_conc=c2, step_hnew=h2, step_ierr=e2)
+    print("wrote", out, out.stat().st_size, "bytes; steps", i1[2])
+
+
+if __name__ == "__main__":
+    make("fullchem", [0, 1, 2, 3, 2048, 2049, 2050, 2051])
