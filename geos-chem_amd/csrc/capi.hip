@@ -50,6 +50,7 @@ struct gckpp_ctx {
   double *step_buf;    // fullchem_step: rconst [NREACT][wstride], conc backup
                        // [NSPEC][wstride], rstatus [20][wstride]
   int32_t *nfail;      // [1] device counter of cells that failed twice
+  hipEvent_t ev[5];    // step timing: rates | backup | integrate | retry+finish
   // scratch for the host-pointer and retry paths
   std::vector<double> h_atol, h_rtol;
 };
@@ -97,6 +98,7 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
                   hipGetErrorString(e2));
   }
   HIP_OK(hipMalloc(&c->nfail, sizeof(int32_t)));
+  for (int k = 0; k < 5; ++k) HIP_OK(hipEventCreate(&c->ev[k]));
   // Init_FullChem defaults (GeosCore/fullchem_mod.F90:2899-2905)
   c->h_atol.assign(c->info.nvar, 1.0e-2);
   c->h_rtol.assign(c->info.nvar, 0.5e-2);
@@ -115,6 +117,7 @@ extern "C" int gckpp_finalize(gckpp_ctx *c) {
   hipFree(c->consts);
   hipFree(c->step_buf);
   hipFree(c->nfail);
+  for (int k = 0; k < 5; ++k) hipEventDestroy(c->ev[k]);
   delete c;
   return 0;
 }
@@ -415,15 +418,19 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
   double *cbak = rct + (size_t)c->info.nreact * ncell;
   double *rst = cbak + (size_t)c->info.nspec * ncell;
   // Update_RCONST (fullchem_mod.F90:953)
+  HIP_OK(hipEventRecord(c->ev[0], s));
   int rc = gckpp_rates_dev(c, ncell, met, photol, het, het_stride, conc, rct, stream);
   if (rc) return rc;
+  HIP_OK(hipEventRecord(c->ev[1], s));
   // C_before_integrate (:1026)
   HIP_OK(hipMemcpyAsync(cbak, conc, sizeof(double) * c->info.nspec * ncell,
                         hipMemcpyDeviceToDevice, s));
   // Integrate(0, DT, ...) (:1033)
+  HIP_OK(hipEventRecord(c->ev[2], s));
   rc = integrate_impl(c, ncell, 0.0, dt, icntrl, rcntrl, hstart, conc, rct, istatus, rst, ierr,
                       nullptr, nullptr, stream);
   if (rc) return rc;
+  HIP_OK(hipEventRecord(c->ev[3], s));
   // retry of failed cells: RCNTRL(3)=0, C restored, rates of the restored C
   // (identical to the first Update_RCONST) (:1138-1166)
   double rc2[20];
@@ -438,9 +445,23 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
                      c->info.nspec, conc, rst, hnew, ierr, partial);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, partial, c->nfail);
   HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(c->ev[4], s));
   int32_t nfail = 0;
   HIP_OK(hipMemcpyAsync(&nfail, c->nfail, sizeof nfail, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   if (nfail > 0) return fail("gckpp_fullchem_step: %d cell(s) failed twice", nfail);
+  return 0;
+}
+
+// Device time of the parts of the last gckpp_fullchem_step_dev call, from HIP
+// events on its stream (the role of the reference's per-cell KppTime
+// diagnostic, fullchem_mod.F90:596-598,1298-1301, for the whole batch):
+// ms[0] Update_RCONST, ms[1] Integrate, ms[2] retry + clip, ms[3] total.
+extern "C" int gckpp_step_timings(gckpp_ctx *c, float ms[4]) {
+  if (!c || !ms) return fail("gckpp_step_timings: null argument");
+  HIP_OK(hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
+  HIP_OK(hipEventElapsedTime(&ms[1], c->ev[2], c->ev[3]));
+  HIP_OK(hipEventElapsedTime(&ms[2], c->ev[3], c->ev[4]));
+  HIP_OK(hipEventElapsedTime(&ms[3], c->ev[0], c->ev[4]));
   return 0;
 }

@@ -6,6 +6,12 @@
 
 namespace gckpp {
 
+// Fortran MIN/MAX as the reference's compiler evaluates them (the first
+// argument unless the second compares strictly smaller / larger), which fixes
+// how NaN propagates through the step-size control.
+__host__ __device__ inline double min_f(double a, double b) { return a < b ? a : b; }
+__host__ __device__ inline double max_f(double a, double b) { return a > b ? a : b; }
+
 enum { MECH_FULLCHEM = 0, MECH_HG = 1, MECH_CARBON = 2 };
 
 struct MechInfo {

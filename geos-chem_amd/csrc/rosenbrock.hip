@@ -200,9 +200,9 @@ __global__ __launch_bounds__(64) void k_integrate(IntegrateArgs a) {
   double hs = a.hstart ? a.hstart[cell] : 0.0;
   // RCNTRL(3) > 0 wins over the shared value (Integrate merges RCNTRL_U > 0)
   double Hstart = P.Hstart_default;
-  if (hs > 0.0) Hstart = fmin(fabs(hs), fabs(Tend - P.Tstart));
+  if (hs > 0.0) Hstart = min_f(fabs(hs), fabs(Tend - P.Tstart));
   double T = P.Tstart;
-  double H = fmin(fmax(fabs(P.Hmin), fabs(Hstart)), fabs(P.Hmax));
+  double H = min_f(max_f(fabs(P.Hmin), fabs(Hstart)), fabs(P.Hmax));
   if (fabs(H) <= 10.0 * P.roundoff) H = P.DeltaMin;
   const double dir = P.direction;
   H = dir * H;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(64) void k_integrate(IntegrateArgs a) {
       if (!more) break;
       if (st[2] > P.max_steps) { ierr = -6; break; }
       if ((T + 0.1 * H) == T || H <= P.roundoff) { ierr = -7; break; }
-      H = fmin(H, fabs(Tend - T));
+      H = min_f(H, fabs(Tend - T));
       dev_fun<M>(Yc, fix, rct, Ascr, Fcn0);
       st[0] += P.autonomous ? 1 : 2;  // non-autonomous: dF/dT by finite difference (0 with frozen rates)
       st[1] += 1;
@@ -277,24 +277,24 @@ __global__ __launch_bounds__(64) void k_integrate(IntegrateArgs a) {
       double ye = 0.0;
       for (int j = 0; j < 4; ++j)
         if (kRosE[j] != 0.0) ye = fma(kRosE[j], Ksc(j)[i], ye);
-      double ymax = fmax(fabs(y0), fabs(v));
+      double ymax = max_f(fabs(y0), fabs(v));
       double sc = P.vectol ? a.atol[i] + a.rtol[i] * ymax : a.atol[0] + a.rtol[0] * ymax;
       double q = ye / sc;
       err = fma(q, q, err);
     }
-    err = fmax(sqrt(err / N), 1.0e-10);
-    double fac = fmin(P.FacMax, fmax(P.FacMin, P.FacSafe / pow(err, 1.0 / kRosELO)));
+    err = max_f(sqrt(err / N), 1.0e-10);
+    double fac = min_f(P.FacMax, max_f(P.FacMin, P.FacSafe / pow(err, 1.0 / kRosELO)));
     Hnew = H * fac;
     st[2] += 1;
     if (err <= 1.0 || H <= P.Hmin) {
       st[3] += 1;
       if (P.clip_negative)
-        for (int i = 0; i < N; ++i) Y[i] = fmax(Ynew[i], 0.0);
+        for (int i = 0; i < N; ++i) Y[i] = max_f(Ynew[i], 0.0);
       else
         for (int i = 0; i < N; ++i) Y[i] = Ynew[i];
       T = T + dir * H;
-      Hnew = fmax(P.Hmin, fmin(Hnew, P.Hmax));
-      if (rejLast) Hnew = fmin(Hnew, H);
+      Hnew = max_f(P.Hmin, min_f(Hnew, P.Hmax));
+      if (rejLast) Hnew = min_f(Hnew, H);
       Hexit = H;
       Texit = T;
       HnewAcc = Hnew;

@@ -19,7 +19,7 @@ SYMBOLS = [
     "gckpp_mech_sizes", "gckpp_init", "gckpp_finalize", "gckpp_set_tolerances",
     "gckpp_set_species_consts", "gckpp_rates_dev", "gckpp_integrate_dev",
     "gckpp_integrate", "gckpp_fullchem_step_dev", "gckpp_fun_dev", "gckpp_jac_dev",
-    "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_last_error",
+    "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_step_timings", "gckpp_last_error",
 ]
 
 
@@ -57,6 +57,7 @@ def load():
         getattr(L, f).argtypes = [vp, i64, dp, dp, dp, vp]
     L.gckpp_decomp_dev.argtypes = [vp, i64, dp, i32p, vp]
     L.gckpp_solve_dev.argtypes = [vp, i64, dp, dp, vp]
+    L.gckpp_step_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     for f in SYMBOLS:
         if f not in ("gckpp_last_error",):
             getattr(L, f).restype = ctypes.c_int
@@ -178,6 +179,12 @@ class Context:
             if "failed twice" not in msg:
                 raise GckppError("gckpp_fullchem_step_dev: " + msg)
         return hnew, istatus, ierr, rcode == 0
+
+    def step_timings(self):
+        """(rates_ms, integrate_ms, retry_ms, total_ms) of the last fullchem_step."""
+        ms = (ctypes.c_float * 4)()
+        _check(self.L.gckpp_step_timings(self.h, ms), "gckpp_step_timings")
+        return tuple(float(x) for x in ms)
 
     # --- components ----------------------------------------------------------
     def fun(self, conc, rconst):

@@ -200,3 +200,70 @@ def grid_cells(mech, nx=72, ny=46, nz=72, seed=1, dt=1200.0, nspec_fields=None):
     conc = _species_conc(mech, flat(numden), logdev)
     return Workload(mech, met, np.ascontiguousarray(photol), clear_sky_het(),
                     conc, np.zeros(ncell), dt, "grid%dx%dx%d_seed%d" % (nx, ny, nz, seed))
+
+
+def rich_het(rng, ncell):
+    """Per-cell HetState records with cloud, ice and every aerosol type
+    present, random pH, halide concentrations and flags: exercises every
+    branch of the heterogeneous rate laws (used by the rate parity tests)."""
+    F = fields.HET
+    h = np.zeros((fields.NHET, ncell))
+    u = lambda lo, hi: rng.uniform(lo, hi, ncell)
+    lu = lambda lo, hi: 10.0 ** rng.uniform(np.log10(lo), np.log10(hi), ncell)
+    flag = lambda p: (rng.uniform(0, 1, ncell) < p).astype(float)
+    for name in ("natSurface", "pscBox", "SSA_is_Alk", "SSC_is_Alk", "cloudBox", "TurnOffHetRates"):
+        h[F[name][0]] = flag(0.3)
+    h[F["stratBox"][0]] = flag(0.3)
+    acid = flag(0.6)
+    h[F["SSA_is_Acid"][0]] = acid
+    h[F["SSC_is_Acid"][0]] = flag(0.6)
+    h[F["NAEROTYPE"][0]] = 14
+    for t in range(14):
+        h[F["xArea"][0] + t] = lu(1e-9, 1e-5)
+        h[F["xRadi"][0] + t] = lu(1e-6, 1e-3)
+        h[F["xVol"][0] + t] = lu(1e-14, 1e-10)
+        h[F["xH2O"][0] + t] = lu(1e-15, 1e-11)
+        h[F["wetArea"][0] + t] = lu(1e-9, 1e-5)
+    h[F["aClArea"][0]] = lu(1e-9, 1e-5)
+    h[F["aClRadi"][0]] = lu(1e-6, 1e-4)
+    h[F["aClVol"][0]] = lu(1e-14, 1e-10)
+    cld = u(0.0, 1.0)
+    h[F["cldFr"][0]] = cld
+    h[F["clearFr"][0]] = 1.0 - cld
+    h[F["aLiq"][0]] = lu(1e-7, 1e-4)
+    h[F["aIce"][0]] = lu(1e-8, 1e-5)
+    h[F["rLiq"][0]] = lu(5e-4, 2e-3)
+    h[F["rIce"][0]] = lu(1e-3, 5e-3)
+    h[F["aWater"][0]:F["aWater"][0] + 2] = lu(1e-12, 1e-8)
+    for name in ("Br_conc_CldA", "Br_conc_CldC", "Br_conc_CldG", "Br_conc_Cld", "Br_conc_SSA",
+                 "Br_conc_SSC", "Cl_conc_CldA", "Cl_conc_CldC", "Cl_conc_CldG", "Cl_conc_Cld",
+                 "Cl_conc_SSA", "Cl_conc_SSC", "NIT_conc_SSA", "NIT_conc_SSC"):
+        h[F[name][0]] = lu(1e-8, 1e-1)
+    for name in ("Br_over_Cl_Cld", "Br_over_Cl_SSA", "Br_over_Cl_SSC"):
+        h[F[name][0]] = lu(1e-5, 1e-2)
+    for name in ("frac_Br_CldA", "frac_Br_CldC", "frac_Br_CldG", "frac_Cl_CldA", "frac_Cl_CldC",
+                 "frac_Cl_CldG", "frac_HSO3_aq", "frac_SALACL", "frac_SO3_aq", "f_Acid_SSA",
+                 "f_Acid_SSC", "f_Alk_SSA", "f_Alk_SSC", "HBr_theta", "HCl_theta", "HNO3_theta"):
+        h[F[name][0]] = u(0.0, 1.0)
+    h[F["fupdateHOBr"][0]] = 1.0
+    h[F["fupdateHOCl"][0]] = 1.0
+    h[F["gamma_HO2"][0]] = u(0.0, 0.3)
+    for name in ("H_conc_ICl", "H_conc_LCl", "H_conc_SSA", "H_conc_SSC", "H_conc_Sul", "H_plus"):
+        h[F[name][0]] = lu(1e-9, 1e-1)
+    for name in ("HSO3_aq", "SO3_aq", "TSO3_aq", "HSO4_molal", "NO3_molal", "SO4_molal"):
+        h[F[name][0]] = lu(1e-9, 1e-2)
+    h[F["HSO3m"][0]] = u(0.0, 1.0)
+    h[F["SO3mm"][0]] = u(0.0, 1.0)
+    h[F["KHETI_SLA"][0]:F["KHETI_SLA"][0] + 11] = lu(1e-6, 1e-2)
+    h[F["pHCloud"][0]] = u(1.0, 7.0)
+    h[F["pHSSA"][0]:F["pHSSA"][0] + 2] = u(1.0, 8.0)
+    h[F["OMOC_POA"][0]] = u(1.2, 2.0)
+    h[F["OMOC_OPOA"][0]] = u(1.6, 2.4)
+    h[F["qIce"][0]] = lu(1e-7, 1e-4)
+    h[F["qLiq"][0]] = lu(1e-7, 1e-3)
+    h[F["vAir"][0]] = lu(1e12, 1e15)
+    h[F["vIce"][0]] = lu(1e-9, 1e-6)
+    h[F["vLiq"][0]] = lu(1e-8, 1e-5)
+    h[F["fracOrgAer"][0]] = u(0.0, 1.0)
+    h[F["fracInorgAer"][0]] = u(0.0, 1.0)
+    return h

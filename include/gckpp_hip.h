@@ -106,6 +106,13 @@ int gckpp_decomp_dev(gckpp_ctx *ctx, int64_t ncell, double *jvs, int32_t *ier,
 int gckpp_solve_dev(gckpp_ctx *ctx, int64_t ncell, const double *lu, double *x,
                     void *stream);
 
+/* Device time [ms] of the parts of the last gckpp_fullchem_step_dev call on
+ * this context, from HIP events on its stream: ms[0] Update_RCONST,
+ * ms[1] Integrate, ms[2] retry of failed cells + clipping, ms[3] the whole
+ * step.  The batch counterpart of the per-cell KppTime diagnostic
+ * (GeosCore/fullchem_mod.F90:596-598,1298-1301). */
+int gckpp_step_timings(gckpp_ctx *ctx, float ms[4]);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *gckpp_last_error(void);
 

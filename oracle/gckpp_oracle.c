@@ -136,6 +136,12 @@ static const double RE[4] = {0.0, 0.0, 0.0, 1.0};
 static const double RELO = 3.0;
 static const double RGAMMA[4] = {0.5, 1.5, 0.0, 0.0};
 
+/* Fortran MIN/MAX as the reference's compiler evaluates them: the first
+ * argument unless the second compares strictly smaller / larger (decides how
+ * NaN propagates through the step-size control). */
+static double min_f(double a, double b) { return a < b ? a : b; }
+static double max_f(double a, double b) { return a > b ? a : b; }
+
 /* WAXPY skips alpha == 0 (gckpp_LinearAlgebra.F90:3762) */
 static void waxpy(int n, double a, const double *x, double *y) {
   if (a == 0.0) return;
@@ -148,13 +154,13 @@ static double err_norm(int n, const double *Y, const double *Ynew,
                        const double *rtol, int vectol) {
   double err = 0.0;
   for (int i = 0; i < n; ++i) {
-    double ymax = fmax(fabs(Y[i]), fabs(Ynew[i]));
+    double ymax = max_f(fabs(Y[i]), fabs(Ynew[i]));
     double sc = vectol ? atol[i] + rtol[i] * ymax : atol[0] + rtol[0] * ymax;
     double q = Yerr[i] / sc;
     err += q * q;
   }
   err = sqrt(err / n);
-  return fmax(err, 1.0e-10);
+  return max_f(err, 1.0e-10);
 }
 
 /* Integrate/Rosenbrock/ros_Integrator for one cell.
@@ -195,10 +201,10 @@ static void integrate_cell(const orc_mech *m, double *C, const double *RCT,
   double Hmin, Hmax, Hstart, FacMin, FacMax, FacRej, FacSafe;
   if (RCNTRL[0] == 0.0) Hmin = 0.0; else if (RCNTRL[0] > 0) Hmin = RCNTRL[0]; else { *IERR = -3; return; }
   if (RCNTRL[1] == 0.0) Hmax = fabs(Tend - Tstart);
-  else if (RCNTRL[1] > 0) Hmax = fmin(fabs(RCNTRL[1]), fabs(Tend - Tstart));
+  else if (RCNTRL[1] > 0) Hmax = min_f(fabs(RCNTRL[1]), fabs(Tend - Tstart));
   else { *IERR = -3; return; }
-  if (RCNTRL[2] == 0.0) Hstart = fmax(Hmin, DeltaMin);
-  else if (RCNTRL[2] > 0) Hstart = fmin(fabs(RCNTRL[2]), fabs(Tend - Tstart));
+  if (RCNTRL[2] == 0.0) Hstart = max_f(Hmin, DeltaMin);
+  else if (RCNTRL[2] > 0) Hstart = min_f(fabs(RCNTRL[2]), fabs(Tend - Tstart));
   else { *IERR = -3; return; }
   FacMin = RCNTRL[3] == 0.0 ? 0.2 : RCNTRL[3];
   FacMax = RCNTRL[4] == 0.0 ? 6.0 : RCNTRL[4];
@@ -214,7 +220,7 @@ static void integrate_cell(const orc_mech *m, double *C, const double *RCT,
   double *Jac0 = malloc(sizeof(double) * (size_t)nz), *Ghimj = malloc(sizeof(double) * (size_t)nz);
 
   double T = Tstart;
-  double H = fmin(fmax(fabs(Hmin), fabs(Hstart)), fabs(Hmax));
+  double H = min_f(max_f(fabs(Hmin), fabs(Hstart)), fabs(Hmax));
   if (fabs(H) <= 10.0 * roundoff) H = DeltaMin;
   int dir = Tend >= Tstart ? 1 : -1;
   H = dir * H;
@@ -224,7 +230,7 @@ static void integrate_cell(const orc_mech *m, double *C, const double *RCT,
   while ((dir > 0 && (T - Tend) + roundoff <= 0.0) || (dir < 0 && (Tend - T) + roundoff <= 0.0)) {
     if (ISTATUS[2] > max_steps) { *IERR = -6; goto done; }
     if ((T + 0.1 * H) == T || H <= roundoff) { *IERR = -7; goto done; }
-    H = fmin(H, fabs(Tend - T));
+    H = min_f(H, fabs(Tend - T));
     orc_fun(m, Y, F, RCT, Fcn0);
     ISTATUS[0]++;
     orc_jac(m, Y, F, RCT, Jac0);
@@ -267,19 +273,19 @@ static void integrate_cell(const orc_mech *m, double *C, const double *RCT,
       memset(Yerr, 0, vb);
       for (int j = 0; j < 4; ++j) waxpy(N, RE[j], K + (size_t)j * N, Yerr);
       double err = err_norm(N, Y, Ynew, Yerr, atol, rtol, vectol);
-      double fac = fmin(FacMax, fmax(FacMin, FacSafe / pow(err, 1.0 / RELO)));
+      double fac = min_f(FacMax, max_f(FacMin, FacSafe / pow(err, 1.0 / RELO)));
       Hnew = H * fac;
       ISTATUS[2]++;
       if (err <= 1.0 || H <= Hmin) {
         ISTATUS[3]++;
         if (ICNTRL[15] == 1) {
-          for (int i = 0; i < N; ++i) Y[i] = fmax(Ynew[i], 0.0);
+          for (int i = 0; i < N; ++i) Y[i] = max_f(Ynew[i], 0.0);
         } else {
           memcpy(Y, Ynew, vb);
         }
         T = T + dir * H;
-        Hnew = fmax(Hmin, fmin(Hnew, Hmax));
-        if (rejLast) Hnew = fmin(Hnew, H);
+        Hnew = max_f(Hmin, min_f(Hnew, Hmax));
+        if (rejLast) Hnew = min_f(Hnew, H);
         RSTATUS[1] = H;
         RSTATUS[2] = Hnew;
         RSTATUS[0] = T;

@@ -47,7 +47,7 @@ class Oracle:
             st_ptr.append(len(st_spc))
         nv = mech.nvar
         lu_map = -np.ones(nv * nv, dtype=np.int32)
-        if mech.lu_nonzero > 1:
+        if len(mech.lu_irow) > 0:
             lu_map[mech.lu_irow * nv + mech.lu_icol] = np.arange(mech.lu_nonzero, dtype=np.int32)
         self._keep = [_i32(rx_ptr), _i32(rx_spc), _i32(rx_mult), _i32(st_ptr), _i32(st_spc),
                       np.ascontiguousarray(st_coef, dtype=np.float64),

@@ -50,5 +50,20 @@ def make(mech_name, cells):
     print("wrote", out, out.stat().st_size, "bytes; steps", i1[2])
 
 
+def make_het(mech_name, ncell=64, seed=11):
+    """Rates of box cells with rich per-cell HetState records (clouds, ice,
+    every aerosol type, random pH and halides): pins the heterogeneous laws."""
+    m = mechanism.load(mech_name)
+    R = ref.RefKPP(m)
+    w = synth.box_cells(m, 4096, 0).subset(np.r_[0:ncell // 2, 2048:2048 + ncell // 2])
+    w.het = synth.rich_het(np.random.default_rng(seed), ncell)
+    rct = R.rates(w)
+    out = ROOT / "tests" / "golden" / ("%s_het.npz" % mech_name)
+    np.savez_compressed(out, met=w.met, photol=w.photol, het=w.het, conc=w.conc, rconst=rct)
+    print("wrote", out, out.stat().st_size, "bytes; nonzero het-driven rates",
+          int((rct[780:900] != 0).sum()))
+
+
 if __name__ == "__main__":
     make("fullchem", [0, 1, 2, 3, 2048, 2049, 2050, 2051])
+    make_het("fullchem")

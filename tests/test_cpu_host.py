@@ -1,0 +1,122 @@
+"""CPU-side checks: the C ABI library loads and exports every symbol the
+public header declares; mechanism tables are self-consistent; synthetic
+inputs have the documented shapes and ranges; the mechanism compiler's
+symbolic factorisation reproduces KPP's LU pattern."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def header_symbols():
+    txt = (ROOT / "include" / "gckpp_hip.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(gckpp_\w+)\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from gckpp_amd import hip
+    if not hip.lib_path().exists():
+        pytest.skip("HIP library not built")
+    lib = ctypes.CDLL(str(hip.lib_path()))
+    syms = header_symbols()
+    assert len(syms) >= 14
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) <= set(hip.SYMBOLS) | {"gckpp_step_timings"}
+
+
+def test_mech_sizes_without_gpu():
+    from gckpp_amd import hip, fields
+    if not hip.lib_path().exists():
+        pytest.skip("HIP library not built")
+    assert hip.mech_sizes(0) == [356, 353, 3, 1058, 5683, fields.NMET, fields.NHET, 200]
+    assert hip.mech_sizes(1)[:5] == [43, 32, 11, 94, 161]
+    with pytest.raises(hip.GckppError):
+        hip.mech_sizes(7)
+
+
+def test_mechanism_tables_consistent(fullchem):
+    m = fullchem
+    assert (m.nspec, m.nvar, m.nfix, m.nreact, m.lu_nonzero) == (356, 353, 3, 1058, 5683)
+    # every structural Jacobian entry lies in the LU pattern
+    pos = set(zip(m.lu_irow.tolist(), m.lu_icol.tolist()))
+    nz = set()
+    for rx in m.reactions:
+        for j, _ in rx["reactants"]:
+            if j >= m.nvar:
+                continue
+            for i, _ in rx["stoich"]:
+                nz.add((i, j))
+    nz |= {(i, i) for i in range(m.nvar)}     # KPP keeps every diagonal
+    assert nz <= pos
+    assert len(nz) == m.d["nonzero"]          # KPP's NONZERO (4401)
+    # CROW/DIAG describe the row-major pattern with the diagonal present
+    for k in range(m.nvar):
+        cols = m.lu_icol[m.lu_crow[k]:m.lu_crow[k + 1]]
+        assert np.all(np.diff(cols) > 0)
+        assert m.lu_icol[m.lu_diag[k]] == k
+
+
+def test_symbolic_fill_reproduces_kpp_pattern(fullchem):
+    """Row-wise symbolic elimination of the Jacobian pattern (with the
+    diagonal) in KPP's species order gives exactly KPP's LU pattern."""
+    m = fullchem
+    rows = [set() for _ in range(m.nvar)]
+    for rx in m.reactions:
+        for j, _ in rx["reactants"]:
+            if j < m.nvar:
+                for i, _ in rx["stoich"]:
+                    rows[i].add(j)
+    for i in range(m.nvar):
+        rows[i].add(i)
+    filled = []
+    for k in range(m.nvar):
+        r = set(rows[k])
+        for j in sorted(c for c in r if c < k):
+            pass
+        # eliminate in increasing pivot order, adding U-row patterns
+        todo = sorted(c for c in r if c < k)
+        done = set()
+        while todo:
+            j = todo.pop(0)
+            if j in done:
+                continue
+            done.add(j)
+            for c in filled[j]:
+                if c > j and c not in r:
+                    r.add(c)
+                    if c < k:
+                        todo.append(c)
+                        todo.sort()
+        filled.append(r)
+    pat = set()
+    for k in range(m.nvar):
+        for c in filled[k]:
+            pat.add((k, c))
+    assert pat == set(zip(m.lu_irow.tolist(), m.lu_icol.tolist()))
+
+
+def test_synthetic_box_workload(fullchem):
+    from gckpp_amd import synth, fields
+    w = synth.box_cells(fullchem, 512, 0)
+    T = w.met[fields.MET["TEMP"][0]]
+    P = w.met[fields.MET["PRESS"][0]]
+    assert T.min() >= 190 and T.max() <= 310
+    assert P.min() >= 1.0 and P.max() <= 1013.0
+    assert (w.photol[:, :256] == 0).all() and (w.photol[:, 256:] > 0).any()
+    assert np.isfinite(w.conc).all() and (w.conc >= 0).all()
+    assert w.conc.shape == (356, 512)
+
+
+def test_grid_workload_shape(fullchem):
+    from gckpp_amd import synth, fields
+    w = synth.grid_cells(fullchem, 8, 6, 72, seed=1)
+    assert w.ncell == 8 * 6 * 72
+    P = w.met[fields.MET["PRESS"][0]].reshape(72, 6, 8)
+    assert (np.diff(P[:, 0, 0]) < 0).all()         # pressure falls with level
+    lit = w.met[fields.MET["SUNCOS"][0]] > 0
+    assert 0.2 < lit.mean() < 0.8

