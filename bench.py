@@ -37,7 +37,22 @@ def parse():
     p.add_argument("--grid", default="72x46x72", help="NXxNYxNZ of the synthetic grid")
     p.add_argument("--cpu-sample", type=int, default=8192,
                    help="cells of the same grid timed with the CPU reference (0: skip)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a plain 1 GiB device copy (reference for HBM byte counters)")
     return p.parse_args()
+
+
+def copy_calibration():
+    """A plain device-to-device copy of exactly 1 GiB, five times: the known
+    byte count that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = (1 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 1 << 30, "repeats": 5}), flush=True)
 
 
 def cpu_baseline(mech, w, ic, rc, nsample):
@@ -64,6 +79,8 @@ def cpu_baseline(mech, w, ic, rc, nsample):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the chemistry step has no "
               "cross-GPU communication (cells shard independently), so run --gpus 1",
