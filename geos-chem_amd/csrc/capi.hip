@@ -51,6 +51,9 @@ struct gckpp_ctx {
                        // [NSPEC][wstride], rstatus [20][wstride]
   int32_t *nfail;      // [1] device counter of cells that failed twice
   hipEvent_t ev[5];    // step timing: rates | backup | integrate | retry+finish
+  int solver;          // GCKPP_SOLVER_PLAIN or GCKPP_SOLVER_PERSISTENT
+  unsigned int *counter;  // work-stealing cell counter of the persistent solver
+  int64_t pblocks;     // waves of one persistent launch
   // scratch for the host-pointer and retry paths
   std::vector<double> h_atol, h_rtol;
 };
@@ -79,7 +82,9 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   HIP_OK(hipGetDevice(&c->device));
   c->max_cells = max_cells;
   c->wstride = (max_cells + 63) / 64 * 64;
-  size_t wbytes = sizeof(double) * (size_t)integrate_work_per_cell(c->info) * (size_t)c->wstride;
+  int64_t per_cell = integrate_work_per_cell(c->info);
+  if (persistent_slot_fields(mech_id) > per_cell) per_cell = persistent_slot_fields(mech_id);
+  size_t wbytes = sizeof(double) * (size_t)per_cell * (size_t)c->wstride;
   hipError_t e = hipMalloc(&c->work, wbytes);
   if (e != hipSuccess) {
     delete c;
@@ -99,6 +104,11 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   }
   HIP_OK(hipMalloc(&c->nfail, sizeof(int32_t)));
   for (int k = 0; k < 5; ++k) HIP_OK(hipEventCreate(&c->ev[k]));
+  HIP_OK(hipMalloc(&c->counter, sizeof(unsigned int)));
+  c->solver = GCKPP_SOLVER_PLAIN;
+  c->pblocks = persistent_blocks(mech_id);
+  if (c->pblocks <= 0) c->pblocks = 1024;
+  if (c->pblocks > c->wstride / 64) c->pblocks = c->wstride / 64;
   // Init_FullChem defaults (GeosCore/fullchem_mod.F90:2899-2905)
   c->h_atol.assign(c->info.nvar, 1.0e-2);
   c->h_rtol.assign(c->info.nvar, 0.5e-2);
@@ -118,6 +128,7 @@ extern "C" int gckpp_finalize(gckpp_ctx *c) {
   hipFree(c->step_buf);
   hipFree(c->nfail);
   for (int k = 0; k < 5; ++k) hipEventDestroy(c->ev[k]);
+  hipFree(c->counter);
   delete c;
   return 0;
 }
@@ -224,6 +235,29 @@ static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
     // every cell fails the same option check, as Rosenbrock() would
     hipLaunchKernelGGL(k_fill_status, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s,
                        ncell, ierr, code, istatus, rstatus);
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
+  if (c->solver == GCKPP_SOLVER_PERSISTENT && !only_failed) {
+    // one launch of resident waves stealing cells; any ncell
+    IntegrateArgs a{};
+    a.ncell = ncell;
+    a.conc = conc;
+    a.cstride = ncell;
+    a.rconst = rconst;
+    a.rstride = ncell;
+    a.hstart = hstart;
+    a.atol = c->atol;
+    a.rtol = c->rtol;
+    a.work = c->work;
+    a.prm = p;
+    a.istatus = istatus;
+    a.rstatus = rstatus;
+    a.ierr = ierr;
+    a.sstride = ncell;
+    int64_t nb = (ncell + 63) / 64;
+    if (nb > c->pblocks) nb = c->pblocks;
+    launch_integrate_persistent_mech(c->mech, a, nb, c->counter, s);
     HIP_OK(hipGetLastError());
     return 0;
   }
@@ -463,5 +497,13 @@ extern "C" int gckpp_step_timings(gckpp_ctx *c, float ms[4]) {
   HIP_OK(hipEventElapsedTime(&ms[1], c->ev[2], c->ev[3]));
   HIP_OK(hipEventElapsedTime(&ms[2], c->ev[3], c->ev[4]));
   HIP_OK(hipEventElapsedTime(&ms[3], c->ev[0], c->ev[4]));
+  return 0;
+}
+
+extern "C" int gckpp_set_solver(gckpp_ctx *c, int solver) {
+  if (!c) return fail("gckpp_set_solver: null context");
+  if (solver != GCKPP_SOLVER_PLAIN && solver != GCKPP_SOLVER_PERSISTENT)
+    return fail("gckpp_set_solver: unknown solver %d", solver);
+  c->solver = solver;
   return 0;
 }

@@ -76,6 +76,11 @@ inline int64_t integrate_work_per_cell(const MechInfo &m) {
   return (int64_t)m.lu_nz + 6 * (int64_t)m.nvar + scr;
 }
 void launch_integrate_mech(int mech, const IntegrateArgs &a, hipStream_t s);
+// persistent work-stealing integrator (k_integrate_p)
+int64_t persistent_slot_fields(int mech);  // doubles per lane slot
+int64_t persistent_blocks(int mech);       // resident waves of one launch
+void launch_integrate_persistent_mech(int mech, const IntegrateArgs &a, int64_t nblocks,
+                                      unsigned int *counter, hipStream_t s);
 void launch_comp_mech(int mech, int which, const CompArgs &a, hipStream_t s);
 
 }  // namespace gckpp

@@ -19,8 +19,10 @@ SYMBOLS = [
     "gckpp_mech_sizes", "gckpp_init", "gckpp_finalize", "gckpp_set_tolerances",
     "gckpp_set_species_consts", "gckpp_rates_dev", "gckpp_integrate_dev",
     "gckpp_integrate", "gckpp_fullchem_step_dev", "gckpp_fun_dev", "gckpp_jac_dev",
-    "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_step_timings", "gckpp_last_error",
+    "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_step_timings", "gckpp_set_solver",
+    "gckpp_last_error",
 ]
+SOLVER_PLAIN, SOLVER_PERSISTENT = 0, 1
 
 
 class GckppError(RuntimeError):
@@ -58,6 +60,7 @@ def load():
     L.gckpp_decomp_dev.argtypes = [vp, i64, dp, i32p, vp]
     L.gckpp_solve_dev.argtypes = [vp, i64, dp, dp, vp]
     L.gckpp_step_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.gckpp_set_solver.argtypes = [vp, ctypes.c_int]
     for f in SYMBOLS:
         if f not in ("gckpp_last_error",):
             getattr(L, f).restype = ctypes.c_int
@@ -114,6 +117,9 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_solver(self, solver):
+        _check(self.L.gckpp_set_solver(self.h, int(solver)), "gckpp_set_solver")
 
     def set_tolerances(self, atol, rtol):
         a = np.ascontiguousarray(atol, dtype=np.float64)

@@ -116,10 +116,32 @@ def build_tables(m):
                 lp_src.append(jj)
             le_p1.append(len(lp_dst))
     row_e0.append(len(le_kk))
+    # the same elimination as dot products per LU position (row-Crout order):
+    # JVS(q) for q=(k,c) receives -L(k,j)*U(j,c) for the pivots j < min(c,k)
+    # of row k, in increasing j (the order KppDecomp applies them), then, for
+    # c < k, is divided by the pivot U(c,c).
+    cr_ptr, cr_l, cr_u, cr_div = [0], [], [], []
+    for k in range(nvar if crow else 0):
+        rowpos = {lu_icol[q]: q for q in range(crow[k], crow[k + 1])}
+        lower = [lu_icol[q] for q in range(crow[k], diag[k])]
+        for q in range(crow[k], crow[k + 1]):
+            c = lu_icol[q]
+            for j in lower:
+                if j >= min(c, k):
+                    break
+                # U(j,c) in row j's pattern?
+                for jj in range(diag[j] + 1, crow[j + 1]):
+                    if lu_icol[jj] == c:
+                        cr_l.append(rowpos[j])
+                        cr_u.append(jj)
+                        break
+            cr_ptr.append(len(cr_l))
+            cr_div.append(diag[c] if c < k else -1)
+    assert len(cr_l) == len(lp_dst)
     return dict(rx_s=rx_s, fv_ptr=fv_ptr, fv_r=fv_r, fv_c=fv_c, pt_r=pt_r, pt_m=pt_m,
                 pt_s=pt_s, jv_ptr=jv_ptr, jv_b=jv_b, jv_c=jv_c, le_kk=le_kk, le_dj=le_dj,
                 le_p0=le_p0, le_p1=le_p1, le_row=le_row, lp_dst=lp_dst, lp_src=lp_src,
-                row_e0=row_e0)
+                row_e0=row_e0, cr_ptr=cr_ptr, cr_l=cr_l, cr_u=cr_u, cr_div=cr_div)
 
 
 def emit(name):
@@ -134,7 +156,8 @@ def emit(name):
            "constexpr int NFV = %d, NJV = %d, NLE = %d, NLP = %d;"
            % (len(t["fv_r"]), len(t["jv_b"]), len(t["le_kk"]), len(t["lp_dst"])), ""]
     ints = ["rx_s", "fv_ptr", "fv_r", "pt_r", "pt_s", "jv_ptr", "jv_b", "le_kk", "le_dj",
-            "le_p0", "le_p1", "le_row", "lp_dst", "lp_src", "row_e0"]
+            "le_p0", "le_p1", "le_row", "lp_dst", "lp_src", "row_e0", "cr_ptr", "cr_l", "cr_u",
+            "cr_div"]
     for k in ints:
         out.append(arr("int", k, t[k]))
     for k in ("fv_c", "pt_m", "jv_c"):

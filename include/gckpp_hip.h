@@ -34,6 +34,7 @@ extern "C" {
 typedef struct gckpp_ctx gckpp_ctx;
 
 enum { GCKPP_FULLCHEM = 0, GCKPP_HG = 1, GCKPP_CARBON = 2 };
+enum { GCKPP_SOLVER_PLAIN = 0, GCKPP_SOLVER_PERSISTENT = 1 };
 
 /* Sizes of one mechanism: out[0..7] = NSPEC, NVAR, NFIX, NREACT, LU_NONZERO,
  * NMET, NHET, NPHOTOL (gckpp_Parameters.F90:33-58; per-cell record sizes of
@@ -47,6 +48,15 @@ int gckpp_mech_sizes(int mech_id, int32_t out[8]);
  * to ATOL=1e-2, RTOL=0.5e-2 until gckpp_set_tolerances. */
 int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out);
 int gckpp_finalize(gckpp_ctx *ctx);
+
+/* How Integrate schedules cells (results are the same to roundoff):
+ * GCKPP_SOLVER_PLAIN       one lane per cell, one launch per workspace chunk
+ * GCKPP_SOLVER_PERSISTENT  resident waves that take the next cell from a
+ *                          global counter as soon as a lane's cell is done
+ *                          (no reference counterpart: the reference's OpenMP
+ *                          DYNAMIC,24 schedule plays this role on the CPU,
+ *                          GeosCore/fullchem_mod.F90:528-546) */
+int gckpp_set_solver(gckpp_ctx *ctx, int solver);
 
 /* ATOL(1:NVAR), RTOL(1:NVAR) (gckpp_Global.F90:78-80), host arrays. */
 int gckpp_set_tolerances(gckpp_ctx *ctx, const double *atol, const double *rtol);

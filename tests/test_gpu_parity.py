@@ -129,3 +129,33 @@ def test_integrate_matches_oracle_on_box_workload(ctx, fullchem, torch_dev):
     err = _scaled_err(c, c_o, fullchem)
     print("max scaled err", err.max(), "median steps", np.median(ist.cpu().numpy()[2]))
     assert err.max() <= 1.0
+
+
+@pytest.mark.parametrize("nblocks_cells", [512, 3000])
+def test_persistent_solver_matches_oracle(fullchem, torch_dev, nblocks_cells):
+    """The persistent work-stealing solver (row-Crout LU, per-lane slots)
+    against the C oracle: same per-cell results at the repo tolerance, same
+    step counts; more cells than resident lanes exercises the stealing."""
+    print("case: persistent solver, %d box cells" % nblocks_cells, flush=True)
+    from gckpp_amd import hip, synth
+    from oracle import orc, ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    w = synth.box_cells(fullchem, 4096, 0).subset(np.r_[0:nblocks_cells // 2, 2048:2048 + nblocks_cells // 2])
+    R = ref.RefKPP(fullchem)
+    rct = R.rates(w)
+    ic, rc = ref.fullchem_icntrl()
+    O = orc.Oracle(fullchem)
+    c_o, i_o, r_o, e_o = O.integrate(w.conc, rct, w.hstart, 0.0, 1200.0, ic, rc, nthreads=8)
+    ctx = hip.Context(fullchem, 4096)
+    ctx.set_solver(hip.SOLVER_PERSISTENT)
+    conc = _t(torch_dev, w.conc)
+    ist, rst, ierr = ctx.integrate(conc, _t(torch_dev, rct), _t(torch_dev, w.hstart), 0.0, 1200.0, ic, rc)
+    c = conc.cpu().numpy()
+    assert (ierr.cpu().numpy() == e_o).all()
+    err = _scaled_err(c, c_o, fullchem)
+    print("max scaled err", err.max())
+    assert err.max() <= 1.0
+    assert np.abs(ist.cpu().numpy()[2] - i_o[2]).max() <= 2
+    assert np.abs(rst.cpu().numpy()[0] - 1200.0).max() < 1e-9
+    ctx.close()
