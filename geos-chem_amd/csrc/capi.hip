@@ -12,6 +12,7 @@
 #include "../../include/gckpp_hip.h"
 #include "cell_fields.h"
 #include "gckpp_internal.h"
+#include "mlp.h"
 
 namespace gckpp {
 void launch_rates_mech(int mech, const RatesArgs &a, hipStream_t s);
@@ -53,14 +54,21 @@ struct gckpp_ctx {
   hipEvent_t ev[5];    // step timing: rates | backup | integrate | retry+finish
   int solver;          // GCKPP_SOLVER_PLAIN or GCKPP_SOLVER_PERSISTENT
   unsigned int *counter;  // work-stealing cell counter of the persistent solver
-  int64_t pblocks;     // waves of one persistent launch
+  const int32_t *order;  // persistent solver: queue order of the cells (device), or null
+  float *hlog;         // persistent solver: step-history log (device), or null
+  int hlog_n;
+  int pvariant;        // persistent kernel variant (loads in flight x register budget)
+  int pwpb;            // waves per workgroup of the persistent launch
+  int64_t pwaves;      // waves of one persistent launch (multiple of pwpb)
   // scratch for the host-pointer and retry paths
   std::vector<double> h_atol, h_rtol;
 };
 
 extern "C" const char *gckpp_last_error(void) { return g_err; }
 
-static bool mech_ok(int mech) { return mech == MECH_FULLCHEM || mech == MECH_HG; }
+static bool mech_ok(int mech) {
+  return mech == MECH_FULLCHEM || mech == MECH_HG || mech == MECH_CARBON;
+}
 
 extern "C" int gckpp_mech_sizes(int mech_id, int32_t out[8]) {
   if (!mech_ok(mech_id)) return fail("unknown or unsupported mechanism id %d", mech_id);
@@ -68,6 +76,20 @@ extern "C" int gckpp_mech_sizes(int mech_id, int32_t out[8]) {
   out[0] = m.nspec; out[1] = m.nvar; out[2] = m.nfix; out[3] = m.nreact;
   out[4] = m.lu_nz; out[5] = NMET; out[6] = NHET; out[7] = NPHOTOL;
   g_err[0] = 0;
+  return 0;
+}
+
+// Persistent configuration and its launch size: every resident wave, at
+// most one wave per 64 cells of the workspace, a whole number of workgroups.
+static const int kDefaultPersistVariant = 2, kDefaultPersistWpb = 1;
+static int set_pconfig(gckpp_ctx *c, int variant, int wpb) {
+  int64_t cap = c->wstride / 64 / wpb * wpb;
+  int64_t res = persist_resident_waves(c->mech, variant, wpb);  // PQ_* == MECH_* for one mechanism
+  if (cap <= 0 || res <= 0) return -1;
+  c->pvariant = variant;
+  c->pwpb = wpb;
+  c->pwaves = res < cap ? res / wpb * wpb : cap;
+  if (c->pwaves <= 0) c->pwaves = wpb;
   return 0;
 }
 
@@ -83,7 +105,7 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   c->max_cells = max_cells;
   c->wstride = (max_cells + 63) / 64 * 64;
   int64_t per_cell = integrate_work_per_cell(c->info);
-  if (persistent_slot_fields(mech_id) > per_cell) per_cell = persistent_slot_fields(mech_id);
+  if (persist_slot_fields(mech_id) > per_cell) per_cell = persist_slot_fields(mech_id);
   size_t wbytes = sizeof(double) * (size_t)per_cell * (size_t)c->wstride;
   hipError_t e = hipMalloc(&c->work, wbytes);
   if (e != hipSuccess) {
@@ -106,9 +128,11 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   for (int k = 0; k < 5; ++k) HIP_OK(hipEventCreate(&c->ev[k]));
   HIP_OK(hipMalloc(&c->counter, sizeof(unsigned int)));
   c->solver = GCKPP_SOLVER_PLAIN;
-  c->pblocks = persistent_blocks(mech_id);
-  if (c->pblocks <= 0) c->pblocks = 1024;
-  if (c->pblocks > c->wstride / 64) c->pblocks = c->wstride / 64;
+  if (set_pconfig(c, kDefaultPersistVariant, kDefaultPersistWpb) != 0) {
+    c->pvariant = kDefaultPersistVariant;
+    c->pwpb = 1;
+    c->pwaves = c->wstride / 64 < 1024 ? c->wstride / 64 : 1024;
+  }
   // Init_FullChem defaults (GeosCore/fullchem_mod.F90:2899-2905)
   c->h_atol.assign(c->info.nvar, 1.0e-2);
   c->h_rtol.assign(c->info.nvar, 0.5e-2);
@@ -121,16 +145,14 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
 
 extern "C" int gckpp_finalize(gckpp_ctx *c) {
   if (!c) return 0;
-  hipFree(c->work);
-  hipFree(c->atol);
-  hipFree(c->rtol);
-  hipFree(c->consts);
-  hipFree(c->step_buf);
-  hipFree(c->nfail);
-  for (int k = 0; k < 5; ++k) hipEventDestroy(c->ev[k]);
-  hipFree(c->counter);
+  bool ok = true;
+  for (void *p : {(void *)c->work, (void *)c->atol, (void *)c->rtol, (void *)c->consts,
+                  (void *)c->step_buf, (void *)c->nfail, (void *)c->counter})
+    if (p) ok = hipFree(p) == hipSuccess && ok;
+  for (int k = 0; k < 5; ++k)
+    if (c->ev[k]) ok = hipEventDestroy(c->ev[k]) == hipSuccess && ok;
   delete c;
-  return 0;
+  return ok ? 0 : fail("gckpp_finalize: releasing device memory failed");
 }
 
 extern "C" int gckpp_set_tolerances(gckpp_ctx *c, const double *atol, const double *rtol) {
@@ -167,6 +189,15 @@ static int resolve_options(const gckpp_ctx *c, double tin, double tout, const in
   memset(p, 0, sizeof *p);
   p->Tstart = tin;
   p->Tend = tout;
+  p->neg_mode = ic[15];
+  if (c->mech == MECH_CARBON) {
+    // forward Euler (KPP/carbon/gckpp_Integrator.F90:93-150): no step options
+    if (ic[14] != -1) {
+      snprintf(why, nwhy, "ICNTRL(15)=%d: rates must be frozen during the step (-1)", ic[14]);
+      return -100;
+    }
+    return 0;
+  }
   p->autonomous = ic[0] != 0;
   p->vectol = ic[1] == 0;
   if (!(ic[2] == 0 || ic[2] == 4)) {
@@ -216,6 +247,31 @@ __global__ void k_fill_status(int64_t n, int32_t *ierr, int code, int32_t *ist, 
   }
 }
 
+// One queue of a persistent launch in step mode: the callers' retry for the
+// Rosenbrock mechanisms, the write-back clip except for carbon (whose caller
+// stores C as it is, carbon_Funcs.F90:300-322).
+static void fill_persist_queue(const gckpp_ctx *c, IntegrateArgs &a, int64_t ncell,
+                               const RosParams &p, double *conc, const double *rct,
+                               const double *hstart, double *hnew, int32_t *istatus,
+                               double *rstatus, int32_t *ierr) {
+  a.ncell = ncell;
+  a.conc = conc;
+  a.cstride = ncell;
+  a.rconst = rct;
+  a.rstride = ncell;
+  a.hstart = hstart;
+  a.atol = c->atol;
+  a.rtol = c->rtol;
+  a.prm = p;
+  a.istatus = istatus;
+  a.rstatus = rstatus;
+  a.ierr = ierr;
+  a.sstride = ncell;
+  a.hnew = hnew;
+  a.retry = c->mech != MECH_CARBON;
+  a.clip_out = c->mech != MECH_CARBON;
+}
+
 static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
                           const int32_t icntrl[20], const double rcntrl[20],
                           const double *hstart, double *conc, const double *rconst,
@@ -238,9 +294,10 @@ static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
     HIP_OK(hipGetLastError());
     return 0;
   }
-  if (c->solver == GCKPP_SOLVER_PERSISTENT && !only_failed) {
+  if (c->solver == GCKPP_SOLVER_PERSISTENT && !only_failed && !restore_from) {
     // one launch of resident waves stealing cells; any ncell
-    IntegrateArgs a{};
+    PersistArgs pa{};
+    IntegrateArgs &a = pa.q[0];
     a.ncell = ncell;
     a.conc = conc;
     a.cstride = ncell;
@@ -249,16 +306,23 @@ static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
     a.hstart = hstart;
     a.atol = c->atol;
     a.rtol = c->rtol;
-    a.work = c->work;
     a.prm = p;
     a.istatus = istatus;
     a.rstatus = rstatus;
     a.ierr = ierr;
     a.sstride = ncell;
-    int64_t nb = (ncell + 63) / 64;
-    if (nb > c->pblocks) nb = c->pblocks;
-    launch_integrate_persistent_mech(c->mech, a, nb, c->counter, s);
-    HIP_OK(hipGetLastError());
+    a.order = c->order;
+    a.hlog = c->hlog;
+    a.hlog_n = c->hlog_n;
+    pa.qend[0] = pa.qend[1] = pa.qend[2] = ncell;
+    pa.work = c->work;
+    pa.slot_fields = persist_slot_fields(c->mech);
+    pa.counter = c->counter;
+    int64_t nw = (ncell + 63) / 64;
+    nw = (nw + c->pwpb - 1) / c->pwpb * c->pwpb;
+    if (nw > c->pwaves) nw = c->pwaves;
+    if (launch_persist(c->mech, c->pvariant, c->pwpb, pa, nw, s) != 0)
+      return fail("gckpp_integrate: persistent launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
   }
   // chunks of at most wstride cells; outputs are indexed with the full ncell
@@ -302,6 +366,8 @@ static int comp(gckpp_ctx *c, int which, int64_t ncell, const double *in, const 
   if (!c) return fail("null context");
   if (ncell <= 0) return 0;
   if (ncell > c->wstride) return fail("ncell=%lld exceeds max_cells", (long long)ncell);
+  if (c->mech == MECH_CARBON && which != 0)
+    return fail("the carbon mechanism has no sparse Jacobian (forward Euler)");
   CompArgs a{};
   a.ncell = ncell;
   a.in = in;
@@ -351,23 +417,25 @@ extern "C" int gckpp_integrate(gckpp_ctx *c, int64_t ncell, double tin, double t
       hipMalloc(&dis, 4 * 20 * n) != hipSuccess || hipMalloc(&die, 4 * n) != hipSuccess) {
     rc = fail("gckpp_integrate: device allocation failed");
   } else {
-    hipMemcpy(dc, conc, 8 * nsp * n, hipMemcpyHostToDevice);
-    hipMemcpy(dr, rconst, 8 * nre * n, hipMemcpyHostToDevice);
-    if (hstart) hipMemcpy(dh, hstart, 8 * n, hipMemcpyHostToDevice);
-    else hipMemset(dh, 0, 8 * n);
-    rc = gckpp_integrate_dev(c, ncell, tin, tout, icntrl, rcntrl, dh, dc, dr, dis, drs, die, nullptr);
+    hipError_t e = hipMemcpy(dc, conc, 8 * nsp * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dr, rconst, 8 * nre * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hstart ? hipMemcpy(dh, hstart, 8 * n, hipMemcpyHostToDevice) : hipMemset(dh, 0, 8 * n);
+    if (e != hipSuccess) rc = fail("gckpp_integrate: %s", hipGetErrorString(e));
+    if (rc == 0)
+      rc = gckpp_integrate_dev(c, ncell, tin, tout, icntrl, rcntrl, dh, dc, dr, dis, drs, die,
+                               nullptr);
     if (rc == 0) {
-      hipError_t e = hipDeviceSynchronize();
+      e = hipDeviceSynchronize();
+      if (e == hipSuccess) e = hipMemcpy(conc, dc, 8 * nsp * n, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && istatus) e = hipMemcpy(istatus, dis, 4 * 20 * n, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && rstatus) e = hipMemcpy(rstatus, drs, 8 * 20 * n, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && ierr) e = hipMemcpy(ierr, die, 4 * n, hipMemcpyDeviceToHost);
       if (e != hipSuccess) rc = fail("gckpp_integrate: %s", hipGetErrorString(e));
     }
-    if (rc == 0) {
-      hipMemcpy(conc, dc, 8 * nsp * n, hipMemcpyDeviceToHost);
-      if (istatus) hipMemcpy(istatus, dis, 4 * 20 * n, hipMemcpyDeviceToHost);
-      if (rstatus) hipMemcpy(rstatus, drs, 8 * 20 * n, hipMemcpyDeviceToHost);
-      if (ierr) hipMemcpy(ierr, die, 4 * n, hipMemcpyDeviceToHost);
-    }
   }
-  hipFree(dc); hipFree(dr); hipFree(dh); hipFree(drs); hipFree(dis); hipFree(die);
+  for (void *p : {(void *)dc, (void *)dr, (void *)dh, (void *)drs, (void *)dis, (void *)die})
+    if (p && hipFree(p) != hipSuccess && rc == 0) rc = fail("gckpp_integrate: hipFree failed");
   return rc;
 }
 
@@ -397,17 +465,18 @@ extern "C" int gckpp_rates_dev(gckpp_ctx *c, int64_t ncell, const double *met,
 
 // C = max(C,0) for every species (fullchem_mod.F90:1329-1350), Hnew from
 // RSTATUS(3), and one partial count of twice-failed cells per block.
-__global__ __launch_bounds__(256) void k_step_finish(int64_t n, int nspec, double *conc,
+__global__ __launch_bounds__(256) void k_step_finish(int64_t n, int nspec, int clip, double *conc,
                                                      const double *rst, double *hnew,
                                                      const int32_t *ierr, int32_t *partial) {
   __shared__ int32_t red[256];
   int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int bad = 0;
   if (i < n) {
-    for (int k = 0; k < nspec; ++k) {
-      double v = conc[(int64_t)k * n + i];
-      conc[(int64_t)k * n + i] = v > 0.0 ? v : 0.0;
-    }
+    if (clip)
+      for (int k = 0; k < nspec; ++k) {
+        double v = conc[(int64_t)k * n + i];
+        conc[(int64_t)k * n + i] = v > 0.0 ? v : 0.0;
+      }
     if (hnew) hnew[i] = rst[2 * n + i];
     bad = ierr[i] < 0;
   }
@@ -456,6 +525,49 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
   int rc = gckpp_rates_dev(c, ncell, met, photol, het, het_stride, conc, rct, stream);
   if (rc) return rc;
   HIP_OK(hipEventRecord(c->ev[1], s));
+  if (c->solver == GCKPP_SOLVER_PERSISTENT) {
+    // one launch: Integrate, the retry of failed cells from their original
+    // C with Hstart = 0, the clip and KPPHvalue = RSTATUS(3), per cell
+    RosParams p;
+    char why[256] = "";
+    int code = resolve_options(c, 0.0, dt, icntrl, rcntrl, &p, why, sizeof why);
+    if (code == -100) return fail("gckpp_fullchem_step: %s", why);
+    HIP_OK(hipEventRecord(c->ev[2], s));
+    if (code < 0) {
+      hipLaunchKernelGGL(k_fill_status, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s,
+                         ncell, ierr, code, istatus, (double *)nullptr);
+    } else {
+      PersistArgs pa{};
+      fill_persist_queue(c, pa.q[0], ncell, p, conc, rct, hstart, hnew, istatus, rst, ierr);
+      pa.q[0].order = c->order;
+      pa.q[0].hlog = c->hlog;
+      pa.q[0].hlog_n = c->hlog_n;
+      pa.qend[0] = pa.qend[1] = pa.qend[2] = ncell;
+      pa.work = c->work;
+      pa.slot_fields = persist_slot_fields(c->mech);
+      pa.counter = c->counter;
+      int64_t nw = (ncell + 63) / 64;
+      nw = (nw + c->pwpb - 1) / c->pwpb * c->pwpb;
+      if (nw > c->pwaves) nw = c->pwaves;
+      if (launch_persist(c->mech, c->pvariant, c->pwpb, pa, nw, s) != 0)
+        return fail("gckpp_fullchem_step: persistent launch failed: %s",
+                    hipGetErrorString(hipGetLastError()));
+    }
+    HIP_OK(hipEventRecord(c->ev[3], s));
+    int64_t nb = (ncell + 255) / 256;
+    int32_t *partial = (int32_t *)cbak;
+    hipLaunchKernelGGL(k_step_finish, dim3((unsigned)nb), dim3(256), 0, s, ncell,
+                       c->info.nspec, 0, conc, (const double *)nullptr, (double *)nullptr, ierr,
+                       partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, partial, c->nfail);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(c->ev[4], s));
+    int32_t nfail = 0;
+    HIP_OK(hipMemcpyAsync(&nfail, c->nfail, sizeof nfail, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (nfail > 0) return fail("gckpp_fullchem_step: %d cell(s) failed twice", nfail);
+    return 0;
+  }
   // C_before_integrate (:1026)
   HIP_OK(hipMemcpyAsync(cbak, conc, sizeof(double) * c->info.nspec * ncell,
                         hipMemcpyDeviceToDevice, s));
@@ -476,7 +588,7 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
   int64_t nb = (ncell + 255) / 256;
   int32_t *partial = (int32_t *)(rst + 3 * (size_t)ncell);  // rows 3.. of rstatus are free
   hipLaunchKernelGGL(k_step_finish, dim3((unsigned)nb), dim3(256), 0, s, ncell,
-                     c->info.nspec, conc, rst, hnew, ierr, partial);
+                     c->info.nspec, c->mech != MECH_CARBON, conc, rst, hnew, ierr, partial);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, partial, c->nfail);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(c->ev[4], s));
@@ -505,5 +617,435 @@ extern "C" int gckpp_set_solver(gckpp_ctx *c, int solver) {
   if (solver != GCKPP_SOLVER_PLAIN && solver != GCKPP_SOLVER_PERSISTENT)
     return fail("gckpp_set_solver: unknown solver %d", solver);
   c->solver = solver;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent-kernel configuration and its autotuner.
+extern "C" int gckpp_set_persistent_config(gckpp_ctx *c, int variant, int waves_per_group) {
+  if (!c) return fail("gckpp_set_persistent_config: null context");
+  if (variant < 0 || variant >= PERSIST_VARIANTS)
+    return fail("gckpp_set_persistent_config: variant %d not in [0,%d)", variant, PERSIST_VARIANTS);
+  if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4)
+    return fail("gckpp_set_persistent_config: waves per workgroup must be 1, 2 or 4");
+  if (set_pconfig(c, variant, waves_per_group) != 0)
+    return fail("gckpp_set_persistent_config: variant %d with %d waves per workgroup does not fit",
+                variant, waves_per_group);
+  return 0;
+}
+
+extern "C" int gckpp_set_cell_order(gckpp_ctx *c, const int32_t *order) {
+  if (!c) return fail("gckpp_set_cell_order: null context");
+  c->order = order;
+  return 0;
+}
+
+extern "C" int gckpp_set_step_log(gckpp_ctx *c, float *hlog, int nattempt) {
+  if (!c) return fail("gckpp_set_step_log: null context");
+  if (hlog && nattempt <= 0) return fail("gckpp_set_step_log: nattempt must be positive");
+  c->hlog = hlog;
+  c->hlog_n = hlog ? nattempt : 0;
+  return 0;
+}
+
+extern "C" int gckpp_get_persistent_config(gckpp_ctx *c, int32_t out[5]) {
+  if (!c || !out) return fail("gckpp_get_persistent_config: null argument");
+  int U = 0, W = 0;
+  persist_variant_info(c->pvariant, &U, &W);
+  out[0] = c->pvariant;
+  out[1] = c->pwpb;
+  out[2] = (int32_t)c->pwaves;
+  out[3] = U;
+  out[4] = W;
+  return 0;
+}
+
+// Times the persistent solver on the caller's cells for every variant and
+// workgroup size, keeps the fastest, and leaves conc untouched (each trial
+// integrates a copy).  ms[v*3 + g] receives the time of variant v with
+// 2^g waves per workgroup (-1 where it does not fit).
+extern "C" int gckpp_autotune_dev(gckpp_ctx *c, int64_t ncell, double tin, double tout,
+                                  const int32_t icntrl[20], const double rcntrl[20],
+                                  const double *hstart, const double *conc,
+                                  const double *rconst, float *ms, void *stream) {
+  if (!c) return fail("gckpp_autotune: null context");
+  if (ncell <= 0 || ncell > c->wstride)
+    return fail("gckpp_autotune: ncell must be in [1, max_cells]");
+  if (!conc || !rconst) return fail("gckpp_autotune: null conc or rconst");
+  hipStream_t s = (hipStream_t)stream;
+  double *scratch = c->step_buf + (size_t)c->info.nreact * ncell;  // the conc backup rows
+  const size_t cbytes = sizeof(double) * c->info.nspec * ncell;
+  int32_t *ierr = (int32_t *)(scratch + (size_t)c->info.nspec * ncell);
+  int keep_solver = c->solver, best_v = c->pvariant, best_g = c->pwpb;
+  float best = -1.0f;
+  c->solver = GCKPP_SOLVER_PERSISTENT;
+  int rc = 0;
+  for (int v = 0; v < PERSIST_VARIANTS && rc == 0; ++v) {
+    for (int g = 0; g < 3 && rc == 0; ++g) {
+      float t = -1.0f;
+      if (set_pconfig(c, v, 1 << g) == 0) {
+        // one untimed run of the first configuration warms caches and code
+        for (int rep = (v == 0 && g == 0) ? 0 : 1; rep < 2 && rc == 0; ++rep) {
+          HIP_OK(hipMemcpyAsync(scratch, conc, cbytes, hipMemcpyDeviceToDevice, s));
+          HIP_OK(hipEventRecord(c->ev[0], s));
+          rc = integrate_impl(c, ncell, tin, tout, icntrl, rcntrl, hstart, scratch, rconst,
+                              nullptr, nullptr, ierr, nullptr, nullptr, stream);
+          HIP_OK(hipEventRecord(c->ev[1], s));
+          HIP_OK(hipEventSynchronize(c->ev[1]));
+          HIP_OK(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+        }
+        if (rc == 0 && (best < 0.0f || t < best)) {
+          best = t;
+          best_v = v;
+          best_g = 1 << g;
+        }
+      }
+      if (ms) ms[v * 3 + g] = t;
+    }
+  }
+  c->solver = keep_solver;
+  if (set_pconfig(c, best_v, best_g) != 0) set_pconfig(c, kDefaultPersistVariant, 1);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-mechanism batch: cells of fullchem, Hg and carbon solved by one
+// persistent launch per chemistry step (queues in that order, the heaviest
+// first), with the whole step -- three Update_RCONST launches, the counter
+// reset and the persistent solve with retry, clip and Hstart update --
+// capturable as one graph and replayed step after step.
+struct gckpp_batch {
+  gckpp_ctx *ctx[3];      // fullchem, Hg, carbon (null: absent)
+  int variant, wpb;
+  int64_t nwaves, slot_fields;
+  double *work;           // [cap_waves][slot_fields][64]
+  int64_t cap_waves;
+  unsigned int *counter;
+  hipStream_t cap_stream; // stream the graph is captured on
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+static int batch_config(gckpp_batch *b, int variant, int wpb, int64_t cap_waves) {
+  int64_t res = persist_resident_waves(PQ_MIXED, variant, wpb);
+  if (res <= 0) return -1;
+  int64_t nw = res / wpb * wpb;
+  if (cap_waves > 0 && nw > cap_waves) nw = cap_waves / wpb * wpb;
+  if (nw <= 0) return -1;
+  b->variant = variant;
+  b->wpb = wpb;
+  b->nwaves = nw;
+  return 0;
+}
+
+extern "C" int gckpp_batch_init(gckpp_ctx *fullchem, gckpp_ctx *hg, gckpp_ctx *carbon,
+                                gckpp_batch **out) {
+  if (!out) return fail("gckpp_batch_init: null output pointer");
+  *out = nullptr;
+  gckpp_ctx *cs[3] = {fullchem, hg, carbon};
+  const int want[3] = {MECH_FULLCHEM, MECH_HG, MECH_CARBON};
+  for (int q = 0; q < 3; ++q)
+    if (cs[q] && cs[q]->mech != want[q])
+      return fail("gckpp_batch_init: context %d is not of the expected mechanism", q);
+  if (!fullchem && !hg && !carbon) return fail("gckpp_batch_init: no context given");
+  auto *b = new gckpp_batch();
+  for (int q = 0; q < 3; ++q) b->ctx[q] = cs[q];
+  b->slot_fields = persist_slot_fields(PQ_MIXED);
+  if (batch_config(b, kDefaultPersistVariant, 1, 0) != 0) {
+    delete b;
+    return fail("gckpp_batch_init: cannot size the persistent launch");
+  }
+  // workspace for the largest configuration (every resident wave of 1-wave groups)
+  size_t wb = sizeof(double) * (size_t)b->nwaves * 64 * (size_t)b->slot_fields;
+  for (int v = 0; v < PERSIST_VARIANTS; ++v) {
+    int64_t r = persist_resident_waves(PQ_MIXED, v, 1);
+    size_t x = sizeof(double) * (size_t)r * 64 * (size_t)b->slot_fields;
+    if (x > wb) wb = x;
+  }
+  hipError_t e = hipMalloc(&b->work, wb);
+  if (e != hipSuccess) {
+    delete b;
+    return fail("gckpp_batch_init: cannot allocate %.2f GB of slots: %s", wb / 1e9,
+                hipGetErrorString(e));
+  }
+  b->cap_waves = (int64_t)(wb / (sizeof(double) * 64 * (size_t)b->slot_fields));
+  HIP_OK(hipMalloc(&b->counter, sizeof(unsigned int)));
+  HIP_OK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+  *out = b;
+  return 0;
+}
+
+// One part of a batch step: the cells of one mechanism (see gckpp_hip.h).
+static int batch_enqueue(gckpp_batch *b, const gckpp_part parts[3], double dt, hipStream_t s) {
+  PersistArgs pa{};
+  int64_t end = 0;
+  int nq = 0;
+  for (int q = 0; q < 3; ++q) {
+    const gckpp_part &P = parts[q];
+    gckpp_ctx *c = b->ctx[q];
+    int64_t n = (c && P.ncell > 0) ? P.ncell : 0;
+    if (n > 0) {
+      if (n > c->wstride)
+        return fail("gckpp_batch: part %d has %lld cells, more than its context's max_cells", q,
+                    (long long)n);
+      if (!P.conc || !P.hstart || !P.met || !P.photol || !P.het)
+        return fail("gckpp_batch: part %d: null argument", q);
+      RosParams p;
+      char why[256] = "";
+      int code = resolve_options(c, 0.0, dt, P.icntrl, P.rcntrl, &p, why, sizeof why);
+      if (code == -100) return fail("gckpp_batch: part %d: %s", q, why);
+      if (code < 0) return fail("gckpp_batch: part %d: options give IERR %d", q, code);
+      double *rct = c->step_buf;
+      int rc = gckpp_rates_dev(c, n, P.met, P.photol, P.het, P.het_stride, P.conc, rct, s);
+      if (rc) return rc;
+      fill_persist_queue(c, pa.q[q], n, p, P.conc, rct, P.hstart, P.hstart, P.istatus, nullptr,
+                         P.ierr);
+      pa.q[q].order = P.order;
+      ++nq;
+    }
+    end += n;
+    pa.qend[q] = end;
+  }
+  if (end == 0) return 0;
+  pa.work = b->work;
+  pa.slot_fields = b->slot_fields;
+  pa.counter = b->counter;
+  int64_t nw = (end + 63) / 64;
+  nw = (nw + b->wpb - 1) / b->wpb * b->wpb;
+  if (nw > b->nwaves) nw = b->nwaves;
+  if (launch_persist(PQ_MIXED, b->variant, b->wpb, pa, nw, s) != 0)
+    return fail("gckpp_batch: persistent launch failed: %s", hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+extern "C" int gckpp_batch_step_dev(gckpp_batch *b, const gckpp_part parts[3], double dt,
+                                    void *stream) {
+  if (!b || !parts) return fail("gckpp_batch_step: null argument");
+  return batch_enqueue(b, parts, dt, (hipStream_t)stream);
+}
+
+extern "C" int gckpp_batch_capture(gckpp_batch *b, const gckpp_part parts[3], double dt) {
+  if (!b || !parts) return fail("gckpp_batch_capture: null argument");
+  if (b->exec) {
+    HIP_OK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    HIP_OK(hipGraphDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  HIP_OK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+  int rc = batch_enqueue(b, parts, dt, b->cap_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail("gckpp_batch_capture: %s", hipGetErrorString(e));
+  b->graph = g;
+  e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) return fail("gckpp_batch_capture: instantiate: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" int gckpp_batch_replay(gckpp_batch *b, void *stream) {
+  if (!b) return fail("gckpp_batch_replay: null batch");
+  if (!b->exec) return fail("gckpp_batch_replay: nothing captured");
+  HIP_OK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return 0;
+}
+
+extern "C" int gckpp_batch_set_config(gckpp_batch *b, int variant, int waves_per_group) {
+  if (!b) return fail("gckpp_batch_set_config: null batch");
+  if (variant < 0 || variant >= PERSIST_VARIANTS ||
+      (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4))
+    return fail("gckpp_batch_set_config: bad variant or waves per workgroup");
+  if (batch_config(b, variant, waves_per_group, b->cap_waves) != 0)
+    return fail("gckpp_batch_set_config: configuration does not fit");
+  return 0;
+}
+
+extern "C" int gckpp_batch_finalize(gckpp_batch *b) {
+  if (!b) return 0;
+  bool ok = true;
+  if (b->exec) ok = hipGraphExecDestroy(b->exec) == hipSuccess && ok;
+  if (b->graph) ok = hipGraphDestroy(b->graph) == hipSuccess && ok;
+  if (b->cap_stream) ok = hipStreamDestroy(b->cap_stream) == hipSuccess && ok;
+  if (b->work) ok = hipFree(b->work) == hipSuccess && ok;
+  if (b->counter) ok = hipFree(b->counter) == hipSuccess && ok;
+  delete b;
+  return ok ? 0 : fail("gckpp_batch_finalize: releasing device resources failed");
+}
+
+// ---------------------------------------------------------------------------
+// Step-size MLP (mlp.hip): inference for the integrator's Hstart and
+// training on logged step histories.
+struct gckpp_mlp {
+  MlpDesc desc;
+  int64_t max_batch;
+  float *params, *grad, *adam_m, *adam_v;  // [MLP_NPARAM] each
+  unsigned short *packed;                  // [MLP_NPACK] bf16
+  bool packed_ok;
+  int step;
+  float *act, *dO, *dA, *lossp;            // training scratch for max_batch rows
+};
+
+extern "C" int gckpp_mlp_init(int64_t max_batch, const int32_t jslot[12], const int32_t spc[24],
+                              gckpp_mlp **out) {
+  if (!out || !jslot || !spc) return fail("gckpp_mlp_init: null argument");
+  *out = nullptr;
+  if (max_batch <= 0) return fail("gckpp_mlp_init: max_batch must be positive");
+  auto *m = new gckpp_mlp();
+  for (int k = 0; k < MLP_NJ; ++k) m->desc.jslot[k] = jslot[k];
+  for (int k = 0; k < MLP_NS; ++k) m->desc.spc[k] = spc[k];
+  for (int k = 0; k < MLP_F; ++k) {
+    m->desc.mean[k] = 0.f;
+    m->desc.rstd[k] = 1.f;
+  }
+  m->desc.out_mu = 0.f;
+  m->desc.out_sigma = 1.f;
+  m->max_batch = max_batch;
+  size_t np = sizeof(float) * MLP_NPARAM;
+  size_t na = sizeof(float) * (size_t)max_batch * (6 * MLP_H + MLP_O);
+  if (hipMalloc(&m->params, np) != hipSuccess || hipMalloc(&m->grad, np) != hipSuccess ||
+      hipMalloc(&m->adam_m, np) != hipSuccess || hipMalloc(&m->adam_v, np) != hipSuccess ||
+      hipMalloc(&m->packed, sizeof(unsigned short) * MLP_NPACK) != hipSuccess ||
+      hipMalloc(&m->act, na) != hipSuccess ||
+      hipMalloc(&m->dO, sizeof(float) * (size_t)max_batch * MLP_O) != hipSuccess ||
+      hipMalloc(&m->dA, sizeof(float) * (size_t)max_batch * MLP_H) != hipSuccess ||
+      hipMalloc(&m->lossp, sizeof(float) * (size_t)((max_batch + 255) / 256)) != hipSuccess) {
+    delete m;
+    return fail("gckpp_mlp_init: device allocation failed");
+  }
+  HIP_OK(hipMemset(m->params, 0, np));
+  HIP_OK(hipMemset(m->adam_m, 0, np));
+  HIP_OK(hipMemset(m->adam_v, 0, np));
+  m->packed_ok = false;
+  *out = m;
+  return 0;
+}
+
+extern "C" int gckpp_mlp_set_norm(gckpp_mlp *m, const float mean[48], const float stdev[48],
+                                  float out_mu, float out_sigma) {
+  if (!m || !mean || !stdev) return fail("gckpp_mlp_set_norm: null argument");
+  for (int k = 0; k < MLP_F; ++k) {
+    if (!(stdev[k] > 0.f)) return fail("gckpp_mlp_set_norm: std[%d] must be positive", k);
+    m->desc.mean[k] = mean[k];
+    m->desc.rstd[k] = 1.f / stdev[k];
+  }
+  if (!(out_sigma > 0.f)) return fail("gckpp_mlp_set_norm: out_sigma must be positive");
+  m->desc.out_mu = out_mu;
+  m->desc.out_sigma = out_sigma;
+  return 0;
+}
+
+extern "C" int gckpp_mlp_set_params(gckpp_mlp *m, const float *params) {
+  if (!m || !params) return fail("gckpp_mlp_set_params: null argument");
+  HIP_OK(hipMemcpy(m->params, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+  size_t np = sizeof(float) * MLP_NPARAM;
+  HIP_OK(hipMemset(m->adam_m, 0, np));
+  HIP_OK(hipMemset(m->adam_v, 0, np));
+  m->step = 0;
+  m->packed_ok = false;
+  return 0;
+}
+
+extern "C" int gckpp_mlp_get_params(gckpp_mlp *m, float *params) {
+  if (!m || !params) return fail("gckpp_mlp_get_params: null argument");
+  HIP_OK(hipMemcpy(params, m->params, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int gckpp_mlp_features_dev(gckpp_mlp *m, int64_t ncell, const double *met,
+                                      const double *photol, const double *conc, float *X,
+                                      void *stream) {
+  if (!m || !met || !photol || !conc || !X) return fail("gckpp_mlp_features: null argument");
+  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_mlp_features: negative ncell");
+  mlp_features(m->desc, ncell, met, photol, conc, X, (hipStream_t)stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gckpp_mlp_predict_dev(gckpp_mlp *m, int64_t ncell, const double *met,
+                                     const double *photol, const double *conc, double safety,
+                                     double hmin, double hmax, double *h0, int32_t *cls,
+                                     void *stream) {
+  if (!m || !met || !photol || !conc) return fail("gckpp_mlp_predict: null argument");
+  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_mlp_predict: negative ncell");
+  if (!(hmin > 0.0) || !(hmax >= hmin) || !(safety > 0.0))
+    return fail("gckpp_mlp_predict: need 0 < hmin <= hmax and safety > 0");
+  hipStream_t s = (hipStream_t)stream;
+  if (!m->packed_ok) {
+    mlp_pack(m->params, m->packed, s);
+    m->packed_ok = true;
+  }
+  mlp_predict(m->desc, ncell, met, photol, conc, m->packed, m->params, safety, hmin, hmax, h0, cls, s);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gckpp_mlp_forward_dev(gckpp_mlp *m, int64_t nb, const float *X, float *out,
+                                     void *stream) {
+  if (!m || !X || !out) return fail("gckpp_mlp_forward: null argument");
+  if (nb <= 0 || nb > m->max_batch) return fail("gckpp_mlp_forward: nb must be in [1, max_batch]");
+  hipStream_t s = (hipStream_t)stream;
+  mlp_forward_f32(m->params, nb, X, m->act, s);
+  HIP_OK(hipMemcpyAsync(out, m->act + (size_t)nb * 6 * MLP_H, sizeof(float) * nb * MLP_O,
+                        hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+static int mlp_grad_impl(gckpp_mlp *m, int64_t nb, const float *X, const float *yh,
+                         const int32_t *ycls, float wh, float wc, float *loss, hipStream_t s) {
+  if (!m || !X || !yh || !ycls) return fail("gckpp_mlp_grad: null argument");
+  if (nb <= 0 || nb > m->max_batch) return fail("gckpp_mlp_grad: nb must be in [1, max_batch]");
+  mlp_grad_f32(m->params, nb, X, yh, ycls, wh, wc, m->act, m->dO, m->dA, m->grad, m->lossp, s);
+  if (loss) mlp_sum(m->lossp, (nb + 255) / 256, loss, s);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gckpp_mlp_grad_dev(gckpp_mlp *m, int64_t nb, const float *X, const float *yh,
+                                  const int32_t *ycls, float wh, float wc, float *grad,
+                                  float *loss, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  int rc = mlp_grad_impl(m, nb, X, yh, ycls, wh, wc, loss, s);
+  if (rc) return rc;
+  if (grad)
+    HIP_OK(hipMemcpyAsync(grad, m->grad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+extern "C" int gckpp_mlp_train_step_dev(gckpp_mlp *m, int64_t nb, const float *X, const float *yh,
+                                        const int32_t *ycls, float wh, float wc, float lr,
+                                        float *loss, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  int rc = mlp_grad_impl(m, nb, X, yh, ycls, wh, wc, loss, s);
+  if (rc) return rc;
+  m->step += 1;
+  mlp_adam(m->params, m->grad, m->adam_m, m->adam_v, lr, m->step, s);
+  m->packed_ok = false;
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gckpp_mlp_finalize(gckpp_mlp *m) {
+  if (!m) return 0;
+  bool ok = true;
+  for (void *p : {(void *)m->params, (void *)m->grad, (void *)m->adam_m, (void *)m->adam_v,
+                  (void *)m->packed, (void *)m->act, (void *)m->dO, (void *)m->dA,
+                  (void *)m->lossp})
+    if (p) ok = hipFree(p) == hipSuccess && ok;
+  delete m;
+  return ok ? 0 : fail("gckpp_mlp_finalize: releasing device memory failed");
+}
+
+extern "C" int gckpp_order_by_class_dev(int64_t ncell, const int32_t *cls, int32_t *order,
+                                        int32_t *scratch, void *stream) {
+  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_order_by_class: negative ncell");
+  if (!cls || !order || !scratch) return fail("gckpp_order_by_class: null argument");
+  order_by_class(ncell, cls, order, scratch, (hipStream_t)stream);
+  HIP_OK(hipGetLastError());
   return 0;
 }

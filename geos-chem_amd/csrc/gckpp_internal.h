@@ -24,6 +24,7 @@ struct RosParams {
   double Tstart, Tend, Hmin, Hmax, Hstart_default, FacMin, FacMax, FacRej, FacSafe;
   double roundoff, DeltaMin, direction;
   int max_steps, vectol, autonomous, clip_negative;
+  int neg_mode;  // ICNTRL(16) as given (forward Euler's negative handling)
 };
 
 struct IntegrateArgs {
@@ -43,6 +44,23 @@ struct IntegrateArgs {
   const int32_t *only_failed;  // if set, solve only cells with only_failed[cell] < 0
   const double *restore_from;  // if set, solved cells first copy conc from here
   RosParams prm;
+  // persistent solver only
+  double *hnew;          // [ncell] RSTATUS(3) (next Hstart) or null
+  int retry;             // a failed cell is integrated once more with Hstart = 0
+  int clip_out;          // write back max(C, 0) for all NSPEC species
+  const int32_t *order;  // queue position -> cell (a permutation), or null
+  float *hlog;           // [2*hlog_n][ncell] (H, err) of the first attempts, or null
+  int hlog_n;
+};
+
+// One launch of the persistent solver: up to three queues of cells (one per
+// mechanism); global index c < qend[0] is cell c of queue 0, and so on.
+struct PersistArgs {
+  IntegrateArgs q[3];
+  int64_t qend[3];
+  double *work;          // [nblocks][slot_fields][64]
+  int64_t slot_fields;
+  unsigned int *counter; // work-stealing counter, or null (one cell per lane)
 };
 
 struct RatesArgs {
@@ -73,14 +91,17 @@ MechInfo mech_info(int mech);
 // doubles of per-cell workspace the integrator needs
 inline int64_t integrate_work_per_cell(const MechInfo &m) {
   int64_t scr = m.nreact > m.npart ? m.nreact : m.npart;
-  return (int64_t)m.lu_nz + 6 * (int64_t)m.nvar + scr;
+  return (int64_t)(m.lu_nz > 0 ? m.lu_nz : 1) + 6 * (int64_t)m.nvar + scr;
 }
 void launch_integrate_mech(int mech, const IntegrateArgs &a, hipStream_t s);
-// persistent work-stealing integrator (k_integrate_p)
-int64_t persistent_slot_fields(int mech);  // doubles per lane slot
-int64_t persistent_blocks(int mech);       // resident waves of one launch
-void launch_integrate_persistent_mech(int mech, const IntegrateArgs &a, int64_t nblocks,
-                                      unsigned int *counter, hipStream_t s);
+// persistent work-stealing integrator (k_persist): queue sets and variants
+enum { PQ_FULLCHEM = 0, PQ_HG = 1, PQ_CARBON = 2, PQ_MIXED = 3 };
+constexpr int PERSIST_VARIANTS = 5;
+int persist_variant_info(int variant, int *U, int *W);
+int64_t persist_slot_fields(int qset);                   // doubles per lane slot
+int64_t persist_resident_waves(int qset, int variant, int wpb);  // 0 on error
+int launch_persist(int qset, int variant, int wpb, const PersistArgs &pa, int64_t nwaves,
+                   hipStream_t s);
 void launch_comp_mech(int mech, int which, const CompArgs &a, hipStream_t s);
 
 }  // namespace gckpp

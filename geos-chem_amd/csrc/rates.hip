@@ -13,6 +13,9 @@
 #include "hetlaws.h"
 #include "ratelaws.h"
 #include "gen/rates_fullchem.h"
+#include "hglaws.h"
+#include "gen/rates_Hg.h"
+#include "gen/rates_carbon.h"
 
 #define NSPEC_FULLCHEM species_fullchem::NSPEC
 
@@ -63,11 +66,29 @@ __global__ __launch_bounds__(64) void k_rates_fullchem(RatesArgs a) {
   rates_fullchem(E, a.rconst + cell, a.ncell);
 }
 
-bool rates_available(int mech) { return mech == MECH_FULLCHEM; }
+__global__ __launch_bounds__(64) void k_rates_Hg(RatesArgs a) {
+  const int64_t cell = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (cell >= a.ncell) return;
+  rl::RateEnv E = make_env(a, cell, species_Hg::NSPEC);
+  rates_Hg(E, a.rconst + cell, a.ncell);
+}
+
+__global__ __launch_bounds__(64) void k_rates_carbon(RatesArgs a) {
+  const int64_t cell = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (cell >= a.ncell) return;
+  rl::RateEnv E = make_env(a, cell, species_carbon::NSPEC);
+  rates_carbon(E, a.rconst + cell, a.ncell);
+}
+
+bool rates_available(int mech) {
+  return mech == MECH_FULLCHEM || mech == MECH_HG || mech == MECH_CARBON;
+}
 
 void launch_rates_mech(int mech, const RatesArgs &a, hipStream_t s) {
   dim3 grid((unsigned)((a.ncell + 63) / 64));
   if (mech == MECH_FULLCHEM) hipLaunchKernelGGL(k_rates_fullchem, grid, dim3(64), 0, s, a);
+  else if (mech == MECH_HG) hipLaunchKernelGGL(k_rates_Hg, grid, dim3(64), 0, s, a);
+  else if (mech == MECH_CARBON) hipLaunchKernelGGL(k_rates_carbon, grid, dim3(64), 0, s, a);
 }
 
 }  // namespace gckpp

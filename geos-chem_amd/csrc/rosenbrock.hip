@@ -15,9 +15,11 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "gckpp_internal.h"
 #include "gen/mech_Hg.h"
+#include "gen/mech_carbon.h"
 #include "gen/mech_fullchem.h"
 
 namespace gckpp {
@@ -70,6 +72,7 @@ struct CCol {
   };
 GCKPP_MECH_TRAITS(fullchem)
 GCKPP_MECH_TRAITS(Hg)
+GCKPP_MECH_TRAITS(carbon)
 
 // Concentration of species s (variable from y, fixed from the conc array).
 template <class M>
@@ -77,9 +80,46 @@ __device__ inline double spc(const CCol &y, const CCol &fix, int s) {
   return s < M::NVAR ? y[s] : fix[s];
 }
 
+// acc + sum_{t0<=t<t1} w(t)*v(t), added in increasing t.  With U > 1 the
+// operands of U terms are loaded before their FMAs, so a wave keeps up to 2U
+// loads in flight instead of waiting on each term; the sum (and its rounding)
+// is the same for every U.
+template <int U, class W, class V>
+__device__ __forceinline__ double gather_fma(int t0, int t1, double acc, W w, V v) {
+  int t = t0;
+  if constexpr (U >= 8) {
+    for (; t + 8 <= t1; t += 8) {
+      double a[8], b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) { a[u] = w(t + u); b[u] = v(t + u); }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = fma(a[u], b[u], acc);
+    }
+  }
+  if constexpr (U >= 4) {
+    for (; t + 4 <= t1; t += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) { a[u] = w(t + u); b[u] = v(t + u); }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = fma(a[u], b[u], acc);
+    }
+  }
+  if constexpr (U >= 2) {
+    for (; t + 2 <= t1; t += 2) {
+      double a0 = w(t), b0 = v(t), a1 = w(t + 1), b1 = v(t + 1);
+      acc = fma(a0, b0, acc);
+      acc = fma(a1, b1, acc);
+    }
+  }
+  for (; t < t1; ++t) acc = fma(w(t), v(t), acc);
+  return acc;
+}
+
 // A(r) = RCT(r)*prod(reactants) into a[], then Vdot(i) = sum c*A(r).
-template <class M>
+template <class M, int U = 1>
 __device__ void dev_fun(const CCol &y, const CCol &fix, const CCol &rct, Col a, Col vdot) {
+#pragma unroll 2
   for (int r = 0; r < M::NREACT; ++r) {
     double v = rct[r];
 #pragma unroll
@@ -90,20 +130,20 @@ __device__ void dev_fun(const CCol &y, const CCol &fix, const CCol &rct, Col a, 
     }
     a[r] = v;
   }
-  for (int i = 0; i < M::NVAR; ++i) {
-    double acc = 0.0;
-    int t1 = M::fv_ptr(i + 1);
-    for (int t = M::fv_ptr(i); t < t1; ++t) acc = fma(M::fv_c(t), a[M::fv_r(t)], acc);
-    vdot[i] = acc;
-  }
+#pragma unroll 1
+  for (int i = 0; i < M::NVAR; ++i)
+    vdot[i] = gather_fma<U>(M::fv_ptr(i), M::fv_ptr(i + 1), 0.0,
+                            [&](int t) { return M::fv_c(t); },
+                            [&](int t) { return a[M::fv_r(t)]; });
 }
 
 // JVS in LU layout: partials into b[], then each LU entry gathers its terms;
 // out[k] = sign*JVS(k) (+ diag on the diagonal), so sign=-1, diag=1/(H*gamma)
 // builds ros_PrepareMatrix's Ghimj directly.
-template <class M>
+template <class M, int U = 1>
 __device__ void dev_jac(const CCol &y, const CCol &fix, const CCol &rct, Col b, Col out,
                         double sign, double diag) {
+#pragma unroll 2
   for (int p = 0; p < M::NPART; ++p) {
     double v = M::pt_m(p) * rct[M::pt_r(p)];
 #pragma unroll
@@ -114,13 +154,15 @@ __device__ void dev_jac(const CCol &y, const CCol &fix, const CCol &rct, Col b, 
     }
     b[p] = v;
   }
+#pragma unroll 1
   for (int k = 0; k < M::LU_NZ; ++k) {
-    double acc = 0.0;
-    int t1 = M::jv_ptr(k + 1);
-    for (int t = M::jv_ptr(k); t < t1; ++t) acc = fma(M::jv_c(t), b[M::jv_b(t)], acc);
+    double acc = gather_fma<U>(M::jv_ptr(k), M::jv_ptr(k + 1), 0.0,
+                               [&](int t) { return M::jv_c(t); },
+                               [&](int t) { return b[M::jv_b(t)]; });
     out[k] = sign * acc;
   }
   if (diag != 0.0)
+#pragma unroll 4
     for (int i = 0; i < M::NVAR; ++i) out[M::lu_diag(i)] += diag;
 }
 
@@ -151,17 +193,19 @@ __device__ int dev_decomp(Col lu) {
 // pivots j < min(c,k) in increasing j -- the same updates, in the same order,
 // as the row-wise elimination -- then is divided by U(c,c) when c < k.  Each
 // position is written once and its partial sum stays in a register.
-template <class M>
+template <class M, int U = 1>
 __device__ int dev_decomp_crout(Col lu) {
   int ier = 0;
   int q = 0;
+#pragma unroll 1
   for (int k = 0; k < M::NVAR; ++k) {
     if (ier == 0 && fabs(lu[M::lu_diag(k)]) < DBL_MIN) ier = k + 1;
     const int q1 = M::lu_crow(k + 1);
+#pragma unroll 1
     for (; q < q1; ++q) {
-      double acc = lu[q];
-      const int t1 = M::cr_ptr(q + 1);
-      for (int t = M::cr_ptr(q); t < t1; ++t) acc = fma(-lu[M::cr_l(t)], lu[M::cr_u(t)], acc);
+      double acc = gather_fma<U>(M::cr_ptr(q), M::cr_ptr(q + 1), lu[q],
+                                 [&](int t) { return -lu[M::cr_l(t)]; },
+                                 [&](int t) { return lu[M::cr_u(t)]; });
       const int dv = M::cr_div(q);
       if (dv >= 0) acc = acc / lu[dv];
       lu[q] = acc;
@@ -171,19 +215,20 @@ __device__ int dev_decomp_crout(Col lu) {
 }
 
 // KppSolve in place: unit-lower forward then upper backward substitution.
-template <class M>
+template <class M, int U = 1>
 __device__ void dev_solve(const CCol &lu, Col x) {
+#pragma unroll 1
   for (int i = 0; i < M::NVAR; ++i) {
     int k0 = M::lu_crow(i), k1 = M::lu_diag(i);
     if (k0 == k1) continue;
-    double acc = x[i];
-    for (int k = k0; k < k1; ++k) acc = fma(-lu[k], x[M::lu_icol(k)], acc);
-    x[i] = acc;
+    x[i] = gather_fma<U>(k0, k1, x[i], [&](int k) { return -lu[k]; },
+                         [&](int k) { return x[M::lu_icol(k)]; });
   }
+#pragma unroll 1
   for (int i = M::NVAR - 1; i >= 0; --i) {
     int kd = M::lu_diag(i), k1 = M::lu_crow(i + 1);
-    double acc = x[i];
-    for (int k = kd + 1; k < k1; ++k) acc = fma(-lu[k], x[M::lu_icol(k)], acc);
+    double acc = gather_fma<U>(kd + 1, k1, x[i], [&](int k) { return -lu[k]; },
+                               [&](int k) { return x[M::lu_icol(k)]; });
     x[i] = acc / lu[kd];
   }
 }
@@ -351,15 +396,21 @@ __global__ __launch_bounds__(64) void k_integrate(IntegrateArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Persistent, work-stealing Rodas3 integrator.  Each workgroup is one wave;
-// each lane owns a slot of the workspace block of its wave, laid out
-// field-major with the 64 slots fastest ([field][64], one contiguous block per
-// wave), and integrates one cell after another: when its cell reaches Tend
-// (or fails) it writes the cell back and takes the next cell index from a
-// global counter, so lanes of one wave never wait for a slow cell of another
-// lane.  Without a counter each lane integrates exactly one cell (the cell of
-// its global thread index).  The per-cell algorithm is that of k_integrate,
-// with the LU factorisation in row-Crout order.
+// Persistent, work-stealing integrator.  Each workgroup is one wave; each
+// lane owns a slot of its wave's workspace block, laid out field-major with
+// the 64 slots fastest ([field][64], one contiguous block per wave), and
+// integrates one cell after another: one loop iteration is one Rosenbrock
+// attempt, and a lane whose cell has reached Tend (or failed) writes it back
+// and takes the next index from a global counter, so lanes never wait for a
+// slow cell of another lane.  The index space is the concatenation of up to
+// three queues (one per mechanism, in the order given: the heaviest first),
+// so one launch serves cells of mixed mechanisms.  Per cell the algorithm is
+// that of k_integrate with the LU factorisation in row-Crout order; in step
+// mode a failed cell is integrated once more from its original state with
+// Hstart = 0 (the callers' retry, fullchem_mod.F90:1138-1166, with the same
+// rate constants since Update_RCONST of the restored C gives them again),
+// and the write-back clips negatives and stores RSTATUS(3) as the next
+// Hstart.
 template <class M>
 struct SlotLayout {
   static constexpr int SCR = M::NREACT > M::NPART ? M::NREACT : M::NPART;
@@ -373,12 +424,46 @@ struct SlotLayout {
   static constexpr int FIELDS = LU + M::LU_NZ;
 };
 
+// Register state of one lane's cell.
+struct LaneState {
+  double T, H, Hnew, Hexit, Texit, HnewAcc;
+  int st[8];
+  int ierr, tries;
+  bool rejLast, rejMore, newStep;
+};
+
 template <class M>
-__global__ __launch_bounds__(64) void k_integrate_p(IntegrateArgs a, unsigned int *counter) {
+__device__ __forceinline__ void lane_begin(const IntegrateArgs &a, double *blk, int64_t cell,
+                                           bool cold, LaneState &S) {
   using L = SlotLayout<M>;
-  const int lane = threadIdx.x;
+  const RosParams &P = a.prm;
+  #pragma unroll 4
+  for (int s = 0; s < M::NSPEC; ++s) blk[(int64_t)(L::Y + s) * 64] = a.conc[(int64_t)s * a.cstride + cell];
+  #pragma unroll 4
+  for (int r = 0; r < M::NREACT; ++r)
+    blk[(int64_t)(L::R + r) * 64] = a.rconst[(int64_t)r * a.rstride + cell];
+  double hs = (a.hstart && !cold) ? a.hstart[cell] : 0.0;
+  double Hstart = P.Hstart_default;
+  if (hs > 0.0) Hstart = min_f(fabs(hs), fabs(P.Tend - P.Tstart));
+  S.T = P.Tstart;
+  double H = min_f(max_f(fabs(P.Hmin), fabs(Hstart)), fabs(P.Hmax));
+  if (fabs(H) <= 10.0 * P.roundoff) H = P.DeltaMin;
+  S.H = P.direction * H;
+  S.Hnew = S.H;
+  S.Hexit = S.Texit = S.HnewAcc = 0.0;
+  for (int k = 0; k < 8; ++k) S.st[k] = 0;
+  S.ierr = 1;
+  S.rejLast = S.rejMore = false;
+  S.newStep = true;
+}
+
+// One Rodas3 attempt (or the end-of-integration checks); true when the cell
+// is done (S.ierr then holds IERR).
+template <class M, int U>
+__device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, double *blk, int64_t cell,
+                                             LaneState &S) {
+  using L = SlotLayout<M>;
   const int N = M::NVAR;
-  double *blk = a.work + (int64_t)blockIdx.x * (64 * (int64_t)L::FIELDS) + lane;
   auto F = [&](int off) { return Col{blk + (int64_t)off * 64, 64}; };
   auto Fc = [&](int off) { return CCol{blk + (int64_t)off * 64, 64}; };
   Col Y = F(L::Y), Fcn0 = F(L::F0), Ynew = F(L::YN), Ascr = F(L::S), LU = F(L::LU);
@@ -388,153 +473,277 @@ __global__ __launch_bounds__(64) void k_integrate_p(IntegrateArgs a, unsigned in
   const RosParams &P = a.prm;
   const double Tend = P.Tend, dir = P.direction;
 
-  int64_t cell = -1;
-  bool first = true;
-  double T = 0, H = 0, Hnew = 0, Hexit = 0, Texit = 0, HnewAcc = 0;
-  int st[8];
-  int ierr = 1;
-  bool rejLast = false, rejMore = false, newStep = true;
-
-  for (;;) {
-    if (cell < 0) {
-      // take the next cell
-      int64_t c;
-      if (counter) c = (int64_t)atomicAdd(counter, 1u);
-      else c = first ? (int64_t)blockIdx.x * 64 + lane : a.ncell;
-      first = false;
-      if (c >= a.ncell) break;
-      cell = c;
-      for (int s = 0; s < M::NSPEC; ++s) Y[s] = a.conc[(int64_t)s * a.cstride + cell];
-      for (int r = 0; r < M::NREACT; ++r) F(L::R)[r] = a.rconst[(int64_t)r * a.rstride + cell];
-      double hs = a.hstart ? a.hstart[cell] : 0.0;
-      double Hstart = P.Hstart_default;
-      if (hs > 0.0) Hstart = min_f(fabs(hs), fabs(Tend - P.Tstart));
-      T = P.Tstart;
-      H = min_f(max_f(fabs(P.Hmin), fabs(Hstart)), fabs(P.Hmax));
-      if (fabs(H) <= 10.0 * P.roundoff) H = P.DeltaMin;
-      H = dir * H;
-      Hnew = H;
-      Hexit = Texit = HnewAcc = 0.0;
-      for (int k = 0; k < 8; ++k) st[k] = 0;
-      ierr = 1;
-      rejLast = rejMore = false;
-      newStep = true;
-    }
-    bool finished = false;
-    if (newStep) {
-      bool more = (dir > 0) ? ((T - Tend) + P.roundoff <= 0.0) : ((Tend - T) + P.roundoff <= 0.0);
-      if (!more) finished = true;
-      else if (st[2] > P.max_steps) { ierr = -6; finished = true; }
-      else if ((T + 0.1 * H) == T || H <= P.roundoff) { ierr = -7; finished = true; }
-    }
-    if (!finished) {
-      if (newStep) {
-        H = min_f(H, fabs(Tend - T));
-        dev_fun<M>(Yc, Yc, rct, Ascr, Fcn0);
-        st[0] += P.autonomous ? 1 : 2;
-        st[1] += 1;
-      }
-      int nconsec = 0;
-      bool singular = true;
-      while (singular) {
-        dev_jac<M>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * H * kRosGamma1));
-        int ising = dev_decomp_crout<M>(LU);
-        st[5] += 1;
-        if (ising == 0) {
-          singular = false;
-        } else {
-          st[7] += 1;
-          ++nconsec;
-          if (nconsec <= 5) H *= 0.5;
-          else break;
-        }
-      }
-      if (singular) {
-        ierr = -8;
-        finished = true;
-      } else {
-        for (int s = 0; s < 4; ++s) {
-          Col Kc = Ks(s);
-          if (s == 0 || !kRosNewF[s]) {
-            for (int i = 0; i < N; ++i) Kc[i] = Fcn0[i];
-          } else {
-            for (int i = 0; i < N; ++i) {
-              double v = Yc[i];
-              for (int j = 0; j < s; ++j) {
-                double aj = kRosA[s * (s - 1) / 2 + j];
-                if (aj != 0.0) v = fma(aj, Ksc(j)[i], v);
-              }
-              Ynew[i] = v;
-            }
-            dev_fun<M>(Ynewc, Yc, rct, Ascr, Kc);
-            st[0] += 1;
-          }
-          for (int j = 0; j < s; ++j) {
-            double hc = kRosC[s * (s - 1) / 2 + j] / (dir * H);
-            if (hc != 0.0)
-              for (int i = 0; i < N; ++i) Kc[i] = fma(hc, Ksc(j)[i], Kc[i]);
-          }
-          dev_solve<M>(CCol{LU.p, 64}, Kc);
-          st[6] += 1;
-        }
-        double err = 0.0;
-        for (int i = 0; i < N; ++i) {
-          double y0 = Yc[i];
-          double v = y0;
-          for (int j = 0; j < 4; ++j)
-            if (kRosM[j] != 0.0) v = fma(kRosM[j], Ksc(j)[i], v);
-          Ynew[i] = v;
-          double ye = 0.0;
-          for (int j = 0; j < 4; ++j)
-            if (kRosE[j] != 0.0) ye = fma(kRosE[j], Ksc(j)[i], ye);
-          double ymax = max_f(fabs(y0), fabs(v));
-          double sc = P.vectol ? a.atol[i] + a.rtol[i] * ymax : a.atol[0] + a.rtol[0] * ymax;
-          double q = ye / sc;
-          err = fma(q, q, err);
-        }
-        err = max_f(sqrt(err / N), 1.0e-10);
-        double fac = min_f(P.FacMax, max_f(P.FacMin, P.FacSafe / pow(err, 1.0 / kRosELO)));
-        Hnew = H * fac;
-        st[2] += 1;
-        if (err <= 1.0 || H <= P.Hmin) {
-          st[3] += 1;
-          if (P.clip_negative)
-            for (int i = 0; i < N; ++i) Y[i] = max_f(Ynew[i], 0.0);
-          else
-            for (int i = 0; i < N; ++i) Y[i] = Ynew[i];
-          T = T + dir * H;
-          Hnew = max_f(P.Hmin, min_f(Hnew, P.Hmax));
-          if (rejLast) Hnew = min_f(Hnew, H);
-          Hexit = H;
-          Texit = T;
-          HnewAcc = Hnew;
-          rejLast = rejMore = false;
-          H = Hnew;
-          newStep = true;
-        } else {
-          if (rejMore) Hnew = H * P.FacRej;
-          rejMore = rejLast;
-          rejLast = true;
-          H = Hnew;
-          if (st[3] >= 1) st[4] += 1;
-          newStep = false;
-        }
-      }
-    }
-    if (finished) {
-      for (int s = 0; s < M::NVAR; ++s) a.conc[(int64_t)s * a.cstride + cell] = Y[s];
-      if (a.istatus)
-        for (int k = 0; k < 20; ++k) a.istatus[(int64_t)k * a.sstride + cell] = k < 8 ? st[k] : 0;
-      if (a.rstatus) {
-        for (int k = 3; k < 20; ++k) a.rstatus[(int64_t)k * a.sstride + cell] = 0.0;
-        a.rstatus[cell] = Texit;
-        a.rstatus[a.sstride + cell] = Hexit;
-        a.rstatus[2 * a.sstride + cell] = HnewAcc;
-      }
-      if (a.ierr) a.ierr[cell] = ierr;
-      cell = -1;
+  if (S.newStep) {
+    bool more = (dir > 0) ? ((S.T - Tend) + P.roundoff <= 0.0) : ((Tend - S.T) + P.roundoff <= 0.0);
+    if (!more) return true;
+    if (S.st[2] > P.max_steps) { S.ierr = -6; return true; }
+    if ((S.T + 0.1 * S.H) == S.T || S.H <= P.roundoff) { S.ierr = -7; return true; }
+    S.H = min_f(S.H, fabs(Tend - S.T));
+    dev_fun<M, U>(Yc, Yc, rct, Ascr, Fcn0);
+    S.st[0] += P.autonomous ? 1 : 2;
+    S.st[1] += 1;
+  }
+  int nconsec = 0;
+  bool singular = true;
+  while (singular) {
+    dev_jac<M, U>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * S.H * kRosGamma1));
+    int ising = dev_decomp_crout<M, U>(LU);
+    S.st[5] += 1;
+    if (ising == 0) {
+      singular = false;
+    } else {
+      S.st[7] += 1;
+      ++nconsec;
+      if (nconsec <= 5) S.H *= 0.5;
+      else break;
     }
   }
+  if (singular) { S.ierr = -8; return true; }
+  const double H = S.H;
+  for (int s = 0; s < 4; ++s) {
+    Col Kc = Ks(s);
+    if (s == 0 || !kRosNewF[s]) {
+      #pragma unroll 4
+      for (int i = 0; i < N; ++i) Kc[i] = Fcn0[i];
+    } else {
+      #pragma unroll 4
+      for (int i = 0; i < N; ++i) {
+        double v = Yc[i];
+        for (int j = 0; j < s; ++j) {
+          double aj = kRosA[s * (s - 1) / 2 + j];
+          if (aj != 0.0) v = fma(aj, Ksc(j)[i], v);
+        }
+        Ynew[i] = v;
+      }
+      dev_fun<M, U>(Ynewc, Yc, rct, Ascr, Kc);
+      S.st[0] += 1;
+    }
+    for (int j = 0; j < s; ++j) {
+      double hc = kRosC[s * (s - 1) / 2 + j] / (dir * H);
+      if (hc != 0.0)
+        #pragma unroll 4
+        for (int i = 0; i < N; ++i) Kc[i] = fma(hc, Ksc(j)[i], Kc[i]);
+    }
+    dev_solve<M, U>(CCol{LU.p, 64}, Kc);
+    S.st[6] += 1;
+  }
+  double err = 0.0;
+  #pragma unroll 4
+  for (int i = 0; i < N; ++i) {
+    double y0 = Yc[i];
+    double v = y0;
+    for (int j = 0; j < 4; ++j)
+      if (kRosM[j] != 0.0) v = fma(kRosM[j], Ksc(j)[i], v);
+    Ynew[i] = v;
+    double ye = 0.0;
+    for (int j = 0; j < 4; ++j)
+      if (kRosE[j] != 0.0) ye = fma(kRosE[j], Ksc(j)[i], ye);
+    double ymax = max_f(fabs(y0), fabs(v));
+    double sc = P.vectol ? a.atol[i] + a.rtol[i] * ymax : a.atol[0] + a.rtol[0] * ymax;
+    double q = ye / sc;
+    err = fma(q, q, err);
+  }
+  err = max_f(sqrt(err / N), 1.0e-10);
+  double fac = min_f(P.FacMax, max_f(P.FacMin, P.FacSafe / pow(err, 1.0 / kRosELO)));
+  S.Hnew = H * fac;
+  if (a.hlog && S.st[2] < a.hlog_n) {  // step history: (H, err) of the first attempts
+    a.hlog[(int64_t)(2 * S.st[2]) * a.ncell + cell] = (float)H;
+    a.hlog[(int64_t)(2 * S.st[2] + 1) * a.ncell + cell] = (float)err;
+  }
+  S.st[2] += 1;
+  if (err <= 1.0 || H <= P.Hmin) {
+    S.st[3] += 1;
+    if (P.clip_negative)
+      #pragma unroll 4
+      for (int i = 0; i < N; ++i) Y[i] = max_f(Ynew[i], 0.0);
+    else
+      #pragma unroll 4
+      for (int i = 0; i < N; ++i) Y[i] = Ynew[i];
+    S.T = S.T + dir * H;
+    S.Hnew = max_f(P.Hmin, min_f(S.Hnew, P.Hmax));
+    if (S.rejLast) S.Hnew = min_f(S.Hnew, H);
+    S.Hexit = H;
+    S.Texit = S.T;
+    S.HnewAcc = S.Hnew;
+    S.rejLast = S.rejMore = false;
+    S.H = S.Hnew;
+    S.newStep = true;
+  } else {
+    if (S.rejMore) S.Hnew = H * P.FacRej;
+    S.rejMore = S.rejLast;
+    S.rejLast = true;
+    S.H = S.Hnew;
+    if (S.st[3] >= 1) S.st[4] += 1;
+    S.newStep = false;
+  }
+  return false;
+}
+
+// The carbon mechanism's whole integration (one forward-Euler step) on the
+// slot: the Euler kernel's algorithm.
+template <class M>
+__device__ __forceinline__ void lane_euler(const IntegrateArgs &a, double *blk, LaneState &S) {
+  using L = SlotLayout<M>;
+  Col Y{blk + (int64_t)L::Y * 64, 64}, Vd{blk + (int64_t)L::F0 * 64, 64};
+  CCol Yc{Y.p, 64}, rct{blk + (int64_t)L::R * 64, 64};
+  dev_fun<M>(Yc, Yc, rct, Col{blk + (int64_t)L::S * 64, 64}, Vd);
+  const double dt = a.prm.Tend - a.prm.Tstart;
+  bool neg = false;
+  #pragma unroll 4
+  for (int i = 0; i < M::NVAR; ++i) neg = neg || (Yc[i] + Vd[i] * dt < 0.0);
+  if ((a.prm.neg_mode == 2 || a.prm.neg_mode == 3) && neg) {
+    S.ierr = -9;
+    return;
+  }
+  #pragma unroll 4
+  for (int i = 0; i < M::NVAR; ++i) {
+    double v = Yc[i] + Vd[i] * dt;
+    if (a.prm.neg_mode == 1 && v < 0.0) v = 0.0;
+    Y[i] = v;
+  }
+  S.ierr = 1;
+}
+
+template <class M>
+__device__ __forceinline__ void lane_finish(const IntegrateArgs &a, const double *blk,
+                                            int64_t cell, const LaneState &S) {
+  using L = SlotLayout<M>;
+  // the integrator changes only the variable species; the step's write-back
+  // clips every species (the callers' MAX(C,0) over NSPEC)
+  const int nout = a.clip_out ? M::NSPEC : M::NVAR;
+  #pragma unroll 4
+  for (int s = 0; s < nout; ++s) {
+    double v = blk[(int64_t)(L::Y + s) * 64];
+    if (a.clip_out) v = v > 0.0 ? v : 0.0;
+    a.conc[(int64_t)s * a.cstride + cell] = v;
+  }
+  if (a.istatus)
+    for (int k = 0; k < 20; ++k) a.istatus[(int64_t)k * a.sstride + cell] = k < 8 ? S.st[k] : 0;
+  if (a.rstatus) {
+    for (int k = 3; k < 20; ++k) a.rstatus[(int64_t)k * a.sstride + cell] = 0.0;
+    a.rstatus[cell] = S.Texit;
+    a.rstatus[a.sstride + cell] = S.Hexit;
+    a.rstatus[2 * a.sstride + cell] = S.HnewAcc;
+  }
+  if (a.hnew) a.hnew[cell] = S.HnewAcc;
+  if (a.ierr) a.ierr[cell] = S.ierr;
+}
+
+// Calls f(tag) with tag a null pointer of the queue's mechanism type.
+template <class M0, class M1, class M2, class Fn>
+__device__ __forceinline__ void on_queue(int q, Fn &&f) {
+  if constexpr (!std::is_void_v<M2>) {
+    if (q == 2) { f((M2 *)nullptr); return; }
+  }
+  if constexpr (!std::is_void_v<M1>) {
+    if (q == 1) { f((M1 *)nullptr); return; }
+  }
+  f((M0 *)nullptr);
+}
+
+template <class M> struct IsEuler : std::false_type {};
+template <> struct IsEuler<Traits_carbon> : std::true_type {};
+
+// Up to three queues M0, M1, M2 (void: absent); U: loads in flight per
+// gather (see gather_fma); W: waves per SIMD the register allocation must
+// allow (1: unconstrained).
+// A workgroup holds blockDim.x/64 waves (cells per workgroup = blockDim.x);
+// each wave has its own slot block.
+template <int U, int W, class M0, class M1, class M2>
+__global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  double *blk = pa.work + wave * (64 * pa.slot_fields) + lane;
+  constexpr int NQ = std::is_void_v<M1> ? 1 : (std::is_void_v<M2> ? 2 : 3);
+  const int64_t total = pa.qend[NQ - 1];
+  int64_t cell = -1;
+  int q = 0;
+  bool first = true;
+  LaneState S;
+  for (;;) {
+    if (cell < 0) {
+      int64_t c;
+      if (pa.counter) c = (int64_t)atomicAdd(pa.counter, 1u);
+      else c = first ? wave * 64 + lane : total;
+      first = false;
+      if (c >= total) break;
+      q = 0;
+      while (q < NQ - 1 && c >= pa.qend[q]) ++q;
+      cell = c - (q ? pa.qend[q - 1] : 0);
+      if (pa.q[q].order) cell = pa.q[q].order[cell];  // queue order (e.g. costliest first)
+      S.tries = 0;
+      on_queue<M0, M1, M2>(q, [&](auto *tag) {
+        using M = std::remove_pointer_t<decltype(tag)>;
+        lane_begin<M>(pa.q[q], blk, cell, false, S);
+      });
+    }
+    bool done = false;
+    on_queue<M0, M1, M2>(q, [&](auto *tag) {
+      using M = std::remove_pointer_t<decltype(tag)>;
+      if constexpr (IsEuler<M>::value) {
+        lane_euler<M>(pa.q[q], blk, S);
+        done = true;
+      } else {
+        done = lane_attempt<M, U>(pa.q[q], blk, cell, S);
+      }
+    });
+    if (done) {
+      const IntegrateArgs &a = pa.q[q];
+      if (S.ierr < 0 && a.retry && S.tries == 0) {
+        on_queue<M0, M1, M2>(q, [&](auto *tag) {
+          using M = std::remove_pointer_t<decltype(tag)>;
+          lane_begin<M>(a, blk, cell, true, S);
+        });
+        S.tries = 1;
+      } else {
+        on_queue<M0, M1, M2>(q, [&](auto *tag) {
+          using M = std::remove_pointer_t<decltype(tag)>;
+          lane_finish<M>(a, blk, cell, S);
+        });
+        cell = -1;
+      }
+    }
+  }
+}
+
+// Forward Euler over [Tstart, Tend] in one step (the carbon mechanism's
+// integrator, KPP/carbon/gckpp_Integrator.F90:155-215): Y += Fun(Y)*(Tend-Tstart),
+// then ICNTRL(16) decides what a negative result does: 1 sets it to 0; 2
+// leaves Y unchanged and returns IERR = -9; 3 (where the reference STOPs the
+// program) does the same as 2 so the caller sees the cells; any other value
+// keeps it.
+template <class M>
+__global__ __launch_bounds__(64) void k_euler(IntegrateArgs a) {
+  const int64_t cell = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (cell >= a.ncell) return;
+  if (a.only_failed && a.only_failed[cell] >= 0) return;
+  if (a.restore_from)
+    for (int k = 0; k < M::NSPEC; ++k)
+      a.conc[(int64_t)k * a.cstride + cell] = a.restore_from[(int64_t)k * a.cstride + cell];
+  Col Y{a.conc + cell, a.cstride};
+  CCol Yc{a.conc + cell, a.cstride};
+  CCol rct{a.rconst + cell, a.rstride};
+  Col Ascr{a.work + cell, a.wstride};
+  Col Vd{a.work + (int64_t)M::NREACT * a.wstride + cell, a.wstride};
+  dev_fun<M>(Yc, Yc, rct, Ascr, Vd);
+  const double dt = a.prm.Tend - a.prm.Tstart;
+  int ierr = 1;
+  bool neg = false;
+  for (int i = 0; i < M::NVAR; ++i) neg = neg || (Yc[i] + Vd[i] * dt < 0.0);
+  if ((a.prm.neg_mode == 2 || a.prm.neg_mode == 3) && neg) {
+    ierr = -9;
+  } else {
+    for (int i = 0; i < M::NVAR; ++i) {
+      double v = Yc[i] + Vd[i] * dt;
+      if (a.prm.neg_mode == 1 && v < 0.0) v = 0.0;
+      Y[i] = v;
+    }
+  }
+  if (a.istatus)
+    for (int k = 0; k < 20; ++k) a.istatus[(int64_t)k * a.sstride + cell] = 0;
+  if (a.rstatus)
+    for (int k = 0; k < 20; ++k) a.rstatus[(int64_t)k * a.sstride + cell] = 0.0;
+  if (a.ierr) a.ierr[cell] = ierr;
 }
 
 __global__ void k_zero_u32(unsigned int *p) {
@@ -592,41 +801,81 @@ static void launch_comp(int which, const CompArgs &a, hipStream_t s) {
 void launch_integrate_mech(int mech, const IntegrateArgs &a, hipStream_t s) {
   if (mech == MECH_FULLCHEM) launch_integrate<Traits_fullchem>(a, s);
   else if (mech == MECH_HG) launch_integrate<Traits_Hg>(a, s);
+  else if (mech == MECH_CARBON)
+    hipLaunchKernelGGL(k_euler<Traits_carbon>, dim3((unsigned)((a.ncell + 63) / 64)), dim3(64), 0,
+                       s, a);
 }
 
-template <class M>
-static int64_t persist_blocks_for() {
+// Persistent-launch variants: queue sets (one mechanism, or fullchem, Hg
+// and carbon in that order with any queue possibly empty) x U x W.
+template <int U, int W>
+static const void *persist_kernel(int qset) {
+  switch (qset) {
+    case PQ_FULLCHEM: return (const void *)k_persist<U, W, Traits_fullchem, void, void>;
+    case PQ_HG: return (const void *)k_persist<U, W, Traits_Hg, void, void>;
+    case PQ_CARBON: return (const void *)k_persist<U, W, Traits_carbon, void, void>;
+    case PQ_MIXED: return (const void *)k_persist<U, W, Traits_fullchem, Traits_Hg, Traits_carbon>;
+  }
+  return nullptr;
+}
+const void *persist_kernel_variant(int qset, int variant) {
+  switch (variant) {
+    case 0: return persist_kernel<1, 1>(qset);
+    case 1: return persist_kernel<4, 1>(qset);
+    case 2: return persist_kernel<8, 1>(qset);
+    case 3: return persist_kernel<4, 4>(qset);
+    case 4: return persist_kernel<8, 4>(qset);
+  }
+  return nullptr;
+}
+int persist_variant_info(int variant, int *U, int *W) {
+  static const int tab[][2] = {{1, 1}, {4, 1}, {8, 1}, {4, 4}, {8, 4}};
+  if (variant < 0 || variant >= PERSIST_VARIANTS) return -1;
+  *U = tab[variant][0];
+  *W = tab[variant][1];
+  return 0;
+}
+int64_t persist_slot_fields(int qset) {
+  switch (qset) {
+    case PQ_FULLCHEM: return SlotLayout<Traits_fullchem>::FIELDS;
+    case PQ_HG: return SlotLayout<Traits_Hg>::FIELDS;
+    case PQ_CARBON: return SlotLayout<Traits_carbon>::FIELDS;
+    case PQ_MIXED: {
+      int64_t f = SlotLayout<Traits_fullchem>::FIELDS;
+      if (SlotLayout<Traits_Hg>::FIELDS > f) f = SlotLayout<Traits_Hg>::FIELDS;
+      if (SlotLayout<Traits_carbon>::FIELDS > f) f = SlotLayout<Traits_carbon>::FIELDS;
+      return f;
+    }
+  }
+  return 0;
+}
+// Resident waves of one launch with wpb waves per workgroup: CUs x the
+// occupancy the variant's register use allows (0 on error).
+int64_t persist_resident_waves(int qset, int variant, int wpb) {
+  const void *k = persist_kernel_variant(qset, variant);
   int dev = 0, ncu = 0, per = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (!k || hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_integrate_p<M>, 64, 0) != hipSuccess)
-    return 0;
-  return (int64_t)ncu * (per > 0 ? per : 1);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64 * wpb, 0) != hipSuccess) return 0;
+  return (int64_t)ncu * (per > 0 ? per : 1) * wpb;
 }
-int64_t persistent_slot_fields(int mech) {
-  if (mech == MECH_FULLCHEM) return SlotLayout<Traits_fullchem>::FIELDS;
-  if (mech == MECH_HG) return SlotLayout<Traits_Hg>::FIELDS;
-  return 0;
-}
-int64_t persistent_blocks(int mech) {
-  if (mech == MECH_FULLCHEM) return persist_blocks_for<Traits_fullchem>();
-  if (mech == MECH_HG) return persist_blocks_for<Traits_Hg>();
-  return 0;
-}
-// nblocks waves of 64 lanes; counter null: one cell per lane (nblocks*64 >=
-// ncell), else work stealing from *counter (reset to 0 here, by a kernel).
-void launch_integrate_persistent_mech(int mech, const IntegrateArgs &a, int64_t nblocks,
-                                      unsigned int *counter, hipStream_t s) {
-  if (counter) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counter);
-  dim3 grid((unsigned)nblocks);
-  if (mech == MECH_FULLCHEM)
-    hipLaunchKernelGGL(k_integrate_p<Traits_fullchem>, grid, dim3(64), 0, s, a, counter);
-  else if (mech == MECH_HG)
-    hipLaunchKernelGGL(k_integrate_p<Traits_Hg>, grid, dim3(64), 0, s, a, counter);
+// nwaves waves in workgroups of wpb waves; counter null: one cell per lane
+// (nwaves*64 >= cells), else work stealing from *counter (zeroed here by a
+// kernel, so the launch pair can be captured in a graph).
+int launch_persist(int qset, int variant, int wpb, const PersistArgs &pa, int64_t nwaves,
+                   hipStream_t s) {
+  const void *k = persist_kernel_variant(qset, variant);
+  if (!k || nwaves <= 0 || wpb < 1 || wpb > 4 || nwaves % wpb) return -1;
+  if (pa.counter) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.counter);
+  void *args[] = {(void *)&pa};
+  return hipLaunchKernel(k, dim3((unsigned)(nwaves / wpb)), dim3(64 * wpb), args, 0, s) == hipSuccess
+             ? 0 : -1;
 }
 void launch_comp_mech(int mech, int which, const CompArgs &a, hipStream_t s) {
   if (mech == MECH_FULLCHEM) launch_comp<Traits_fullchem>(which, a, s);
   else if (mech == MECH_HG) launch_comp<Traits_Hg>(which, a, s);
+  else if (mech == MECH_CARBON && which == 0)
+    hipLaunchKernelGGL(k_fun<Traits_carbon>, dim3((unsigned)((a.ncell + 63) / 64)), dim3(64), 0, s, a);
 }
 
 MechInfo mech_info(int mech) {
@@ -636,6 +885,9 @@ MechInfo mech_info(int mech) {
     m = {T::NSPEC, T::NVAR, T::NFIX, T::NREACT, T::LU_NZ, T::NPART};
   } else if (mech == MECH_HG) {
     using T = Traits_Hg;
+    m = {T::NSPEC, T::NVAR, T::NFIX, T::NREACT, T::LU_NZ, T::NPART};
+  } else if (mech == MECH_CARBON) {
+    using T = Traits_carbon;
     m = {T::NSPEC, T::NVAR, T::NFIX, T::NREACT, T::LU_NZ, T::NPART};
   }
   return m;

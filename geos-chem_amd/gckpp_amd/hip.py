@@ -20,8 +20,28 @@ SYMBOLS = [
     "gckpp_set_species_consts", "gckpp_rates_dev", "gckpp_integrate_dev",
     "gckpp_integrate", "gckpp_fullchem_step_dev", "gckpp_fun_dev", "gckpp_jac_dev",
     "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_step_timings", "gckpp_set_solver",
+    "gckpp_set_persistent_config", "gckpp_get_persistent_config", "gckpp_autotune_dev",
+    "gckpp_batch_init", "gckpp_batch_set_config", "gckpp_batch_step_dev",
+    "gckpp_batch_capture", "gckpp_batch_replay", "gckpp_batch_finalize",
+    "gckpp_set_cell_order", "gckpp_set_step_log",
+    "gckpp_mlp_init", "gckpp_mlp_set_norm", "gckpp_mlp_set_params", "gckpp_mlp_get_params",
+    "gckpp_mlp_features_dev", "gckpp_mlp_predict_dev", "gckpp_mlp_forward_dev",
+    "gckpp_mlp_grad_dev", "gckpp_mlp_train_step_dev", "gckpp_mlp_finalize",
+    "gckpp_order_by_class_dev",
     "gckpp_last_error",
 ]
+MLP_NPARAM = 41360
+PERSIST_VARIANTS = 5
+
+
+class Part(ctypes.Structure):
+    """gckpp_part: the cells of one mechanism in a batch step."""
+    _fields_ = [("ncell", ctypes.c_int64), ("met", ctypes.c_void_p), ("photol", ctypes.c_void_p),
+                ("het", ctypes.c_void_p), ("het_stride", ctypes.c_int64),
+                ("conc", ctypes.c_void_p), ("hstart", ctypes.c_void_p),
+                ("istatus", ctypes.c_void_p), ("ierr", ctypes.c_void_p),
+                ("icntrl", ctypes.c_int32 * 20), ("rcntrl", ctypes.c_double * 20),
+                ("order", ctypes.c_void_p)]
 SOLVER_PLAIN, SOLVER_PERSISTENT = 0, 1
 
 
@@ -61,6 +81,32 @@ def load():
     L.gckpp_solve_dev.argtypes = [vp, i64, dp, dp, vp]
     L.gckpp_step_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.gckpp_set_solver.argtypes = [vp, ctypes.c_int]
+    L.gckpp_set_persistent_config.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.gckpp_get_persistent_config.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+    L.gckpp_autotune_dev.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double, i32p, dp, dp, dp,
+                                     dp, ctypes.POINTER(ctypes.c_float), vp]
+    L.gckpp_batch_init.argtypes = [vp, vp, vp, ctypes.POINTER(vp)]
+    L.gckpp_batch_set_config.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.gckpp_batch_step_dev.argtypes = [vp, ctypes.POINTER(Part), ctypes.c_double, vp]
+    L.gckpp_batch_capture.argtypes = [vp, ctypes.POINTER(Part), ctypes.c_double]
+    L.gckpp_batch_replay.argtypes = [vp, vp]
+    L.gckpp_batch_finalize.argtypes = [vp]
+    L.gckpp_set_cell_order.argtypes = [vp, vp]
+    L.gckpp_set_step_log.argtypes = [vp, vp, ctypes.c_int]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.gckpp_mlp_init.argtypes = [i64, vp, vp, ctypes.POINTER(vp)]
+    L.gckpp_mlp_set_norm.argtypes = [vp, fp, fp, ctypes.c_float, ctypes.c_float]
+    L.gckpp_mlp_set_params.argtypes = [vp, fp]
+    L.gckpp_mlp_get_params.argtypes = [vp, fp]
+    L.gckpp_mlp_features_dev.argtypes = [vp, i64, dp, dp, dp, vp, vp]
+    L.gckpp_mlp_predict_dev.argtypes = [vp, i64, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, dp, vp, vp]
+    L.gckpp_mlp_forward_dev.argtypes = [vp, i64, vp, vp, vp]
+    L.gckpp_mlp_grad_dev.argtypes = [vp, i64, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp, vp, vp]
+    L.gckpp_mlp_train_step_dev.argtypes = [vp, i64, vp, vp, vp, ctypes.c_float, ctypes.c_float,
+                                           ctypes.c_float, vp, vp]
+    L.gckpp_mlp_finalize.argtypes = [vp]
+    L.gckpp_order_by_class_dev.argtypes = [i64, vp, vp, vp, vp]
     for f in SYMBOLS:
         if f not in ("gckpp_last_error",):
             getattr(L, f).restype = ctypes.c_int
@@ -120,6 +166,42 @@ class Context:
 
     def set_solver(self, solver):
         _check(self.L.gckpp_set_solver(self.h, int(solver)), "gckpp_set_solver")
+
+    def set_persistent_config(self, variant, waves_per_group=1):
+        _check(self.L.gckpp_set_persistent_config(self.h, int(variant), int(waves_per_group)),
+               "gckpp_set_persistent_config")
+
+    def set_cell_order(self, order):
+        """Order (device int32 permutation, or None) in which the persistent
+        solver takes cells; the tensor must outlive its use."""
+        self._order = order
+        _check(self.L.gckpp_set_cell_order(self.h, _ptr(order)), "gckpp_set_cell_order")
+
+    def set_step_log(self, hlog, nattempt=0):
+        """Device float tensor [2*nattempt][ncell] (or None) receiving (H, err)
+        of each cell's first attempts in the next persistent solves."""
+        self._hlog = hlog
+        _check(self.L.gckpp_set_step_log(self.h, _ptr(hlog), int(nattempt)), "gckpp_set_step_log")
+
+    def persistent_config(self):
+        """dict(variant, waves_per_group, waves, loads_in_flight, min_waves_per_simd)."""
+        out = (ctypes.c_int32 * 5)()
+        _check(self.L.gckpp_get_persistent_config(self.h, out), "gckpp_get_persistent_config")
+        return dict(zip(("variant", "waves_per_group", "waves", "loads_in_flight",
+                         "min_waves_per_simd"), list(out)))
+
+    def autotune(self, conc, rconst, hstart, tin, tout, icntrl, rcntrl):
+        """Times every persistent configuration on these cells (conc is not
+        modified), keeps the fastest; returns ms[variant][log2 waves_per_group]."""
+        ms = (ctypes.c_float * (3 * PERSIST_VARIANTS))()
+        ic = np.ascontiguousarray(icntrl, dtype=np.int32)
+        rc = np.ascontiguousarray(rcntrl, dtype=np.float64)
+        _check(self.L.gckpp_autotune_dev(self.h, conc.shape[1], float(tin), float(tout),
+                                         ctypes.c_void_p(ic.ctypes.data),
+                                         ctypes.c_void_p(rc.ctypes.data), _ptr(hstart),
+                                         _ptr(conc), _ptr(rconst), ms, _stream()),
+               "gckpp_autotune_dev")
+        return np.array(list(ms), dtype=np.float32).reshape(PERSIST_VARIANTS, 3)
 
     def set_tolerances(self, atol, rtol):
         a = np.ascontiguousarray(atol, dtype=np.float64)
@@ -218,3 +300,171 @@ class Context:
         _check(self.L.gckpp_solve_dev(self.h, x.shape[1], _ptr(lu), _ptr(out), _stream()),
                "gckpp_solve_dev")
         return out
+
+
+class Batch:
+    """gckpp_batch: one persistent launch per chemistry step over the cells of
+    up to three mechanisms (contexts in the order fullchem, Hg, carbon)."""
+
+    def __init__(self, fullchem=None, hg=None, carbon=None):
+        self.L = load()
+        self.ctxs = [fullchem, hg, carbon]
+        h = ctypes.c_void_p()
+        _check(self.L.gckpp_batch_init(*[c.h if c is not None else None for c in self.ctxs],
+                                       ctypes.byref(h)), "gckpp_batch_init")
+        self.h = h
+        self._parts = None
+
+    def set_config(self, variant, waves_per_group=1):
+        _check(self.L.gckpp_batch_set_config(self.h, int(variant), int(waves_per_group)),
+               "gckpp_batch_set_config")
+
+    @staticmethod
+    def part(met=None, photol=None, het=None, conc=None, hstart=None, icntrl=None, rcntrl=None,
+             istatus=None, ierr=None, order=None):
+        """A gckpp_part over device tensors (conc [NSPEC][n], hstart [n] in/out)."""
+        p = Part()
+        if conc is None:
+            return p
+        n = conc.shape[1]
+        p.ncell = n
+        p.met, p.photol, p.het = _ptr(met), _ptr(photol), _ptr(het)
+        p.het_stride = 0 if het.shape[1] == 1 else n
+        p.conc, p.hstart = _ptr(conc), _ptr(hstart)
+        p.istatus, p.ierr = _ptr(istatus), _ptr(ierr)
+        p.order = _ptr(order)
+        for k in range(20):
+            p.icntrl[k] = int(icntrl[k])
+            p.rcntrl[k] = float(rcntrl[k])
+        return p
+
+    def _arr(self, parts):
+        arr = (Part * 3)(*[parts[k] if k < len(parts) and parts[k] is not None else Part()
+                           for k in range(3)])
+        self._parts = arr          # the graph keeps the pointers; keep the tensors alive too
+        return arr
+
+    def step(self, parts, dt):
+        _check(self.L.gckpp_batch_step_dev(self.h, self._arr(parts), float(dt), _stream()),
+               "gckpp_batch_step_dev")
+
+    def capture(self, parts, dt):
+        _check(self.L.gckpp_batch_capture(self.h, self._arr(parts), float(dt)),
+               "gckpp_batch_capture")
+
+    def replay(self):
+        _check(self.L.gckpp_batch_replay(self.h, _stream()), "gckpp_batch_replay")
+
+    def close(self):
+        if self.h:
+            self.L.gckpp_batch_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def order_by_class(cls):
+    """Cells sorted by class, costliest (3) first, stable: int32 device tensor."""
+    import torch
+    n = cls.shape[0]
+    order = torch.empty((n,), dtype=torch.int32, device=cls.device)
+    scratch = torch.empty((4 * ((n + 255) // 256) + 4,), dtype=torch.int32, device=cls.device)
+    _check(load().gckpp_order_by_class_dev(n, _ptr(cls), _ptr(order), _ptr(scratch), _stream()),
+           "gckpp_order_by_class_dev")
+    return order
+
+
+class MLP:
+    """gckpp_mlp: the step-size / stiffness-class network (bf16 MFMA
+    inference, fp32 training kernels)."""
+
+    def __init__(self, max_batch, jslot, spc):
+        import torch
+        self.L = load()
+        self.torch = torch
+        j = np.ascontiguousarray(jslot, dtype=np.int32)
+        sp = np.ascontiguousarray(spc, dtype=np.int32)
+        assert j.size == 12 and sp.size == 24
+        h = ctypes.c_void_p()
+        _check(self.L.gckpp_mlp_init(int(max_batch), ctypes.c_void_p(j.ctypes.data),
+                                     ctypes.c_void_p(sp.ctypes.data), ctypes.byref(h)),
+               "gckpp_mlp_init")
+        self.h = h
+        self.max_batch = int(max_batch)
+        self._loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+
+    @staticmethod
+    def _f(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def set_norm(self, mean, std, out_mu, out_sigma):
+        m, mp = self._f(mean)
+        s, sp = self._f(std)
+        assert m.size == 48 and s.size == 48
+        _check(self.L.gckpp_mlp_set_norm(self.h, mp, sp, float(out_mu), float(out_sigma)),
+               "gckpp_mlp_set_norm")
+
+    def set_params(self, params):
+        p, pp = self._f(params)
+        assert p.size == MLP_NPARAM
+        _check(self.L.gckpp_mlp_set_params(self.h, pp), "gckpp_mlp_set_params")
+
+    def params(self):
+        out = np.zeros(MLP_NPARAM, dtype=np.float32)
+        _check(self.L.gckpp_mlp_get_params(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "gckpp_mlp_get_params")
+        return out
+
+    def features(self, met, photol, conc):
+        n = conc.shape[1]
+        X = self.torch.empty((n, 48), dtype=self.torch.float32, device=conc.device)
+        _check(self.L.gckpp_mlp_features_dev(self.h, n, _ptr(met), _ptr(photol), _ptr(conc), _ptr(X),
+                                             _stream()), "gckpp_mlp_features_dev")
+        return X
+
+    def predict(self, met, photol, conc, safety=1.0, hmin=1e-5, hmax=1200.0, h0=None, cls=None):
+        T = self.torch
+        n = conc.shape[1]
+        if h0 is None:
+            h0 = T.empty((n,), dtype=T.float64, device=conc.device)
+        if cls is None:
+            cls = T.empty((n,), dtype=T.int32, device=conc.device)
+        _check(self.L.gckpp_mlp_predict_dev(self.h, n, _ptr(met), _ptr(photol), _ptr(conc),
+                                            float(safety), float(hmin), float(hmax), _ptr(h0),
+                                            _ptr(cls), _stream()), "gckpp_mlp_predict_dev")
+        return h0, cls
+
+    def forward(self, X):
+        out = self.torch.empty((X.shape[0], 16), dtype=self.torch.float32, device=X.device)
+        _check(self.L.gckpp_mlp_forward_dev(self.h, X.shape[0], _ptr(X), _ptr(out), _stream()),
+               "gckpp_mlp_forward_dev")
+        return out
+
+    def grad(self, X, yh, ycls, wh=1.0, wc=1.0):
+        g = self.torch.empty((MLP_NPARAM,), dtype=self.torch.float32, device=X.device)
+        _check(self.L.gckpp_mlp_grad_dev(self.h, X.shape[0], _ptr(X), _ptr(yh), _ptr(ycls),
+                                         float(wh), float(wc), _ptr(g), _ptr(self._loss), _stream()),
+               "gckpp_mlp_grad_dev")
+        return g, self._loss
+
+    def train_step(self, X, yh, ycls, lr, wh=1.0, wc=1.0):
+        _check(self.L.gckpp_mlp_train_step_dev(self.h, X.shape[0], _ptr(X), _ptr(yh), _ptr(ycls),
+                                               float(wh), float(wc), float(lr), _ptr(self._loss),
+                                               _stream()), "gckpp_mlp_train_step_dev")
+        return self._loss
+
+    def close(self):
+        if self.h:
+            self.L.gckpp_mlp_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -123,6 +123,33 @@ def _species_conc(mech, numden, logdev):
     return conc
 
 
+def _carbon_inputs(mech, met, conc, rng):
+    """The carbon mechanism's per-cell rate inputs, as
+    carbon_ComputeRateConstants sets them (KPP/carbon/carbon_Funcs.F90:171-245):
+    in tropospheric cells (TROP=1) fixed OH and Cl, k_Trop(1) = P(CO) from CH4,
+    k_Trop(2) = k(CO+OH), k_Trop(3) = P(CO) from NMVOC; in stratospheric
+    cells k_Strat(1), k_Strat(3) first-order CH4 and CO losses and k_Strat(2)
+    = P(CO) there; the dummy species are 1 (:102-109)."""
+    M = fields.MET
+    n = conc.shape[1]
+    trop = met[M["TROP"][0]] > 0.0
+    lu = lambda lo, hi: 10.0 ** rng.uniform(np.log10(lo), np.log10(hi), n)
+    idx = {s: i for i, s in enumerate(mech.species)}
+    for d in ("DummyCH4trop", "DummyCH4strat", "DummyNMVOC"):
+        conc[idx[d]] = 1.0
+    conc[idx["FixedOH"]] = np.where(trop, lu(1e5, 5e6), 1.0)
+    conc[idx["FixedCl"]] = np.where(trop, lu(1e2, 1e4), 1.0)
+    kt, ks = M["K_TROP"][0], M["K_STRAT"][0]
+    met[kt:kt + 5] = 0.0
+    met[ks:ks + 5] = 0.0
+    met[kt + 0] = np.where(trop, lu(1e1, 1e4), 0.0)
+    met[kt + 1] = np.where(trop, lu(1.5e-13, 3e-13), 0.0)
+    met[kt + 2] = np.where(trop, lu(1e1, 1e4), 0.0)
+    met[ks + 0] = np.where(trop, 0.0, lu(1e-10, 1e-7))
+    met[ks + 1] = np.where(trop, 0.0, lu(1e0, 1e3))
+    met[ks + 2] = np.where(trop, 0.0, lu(1e-10, 1e-7))
+
+
 def box_cells(mech, ncell=4096, seed=0, dt=1200.0):
     rng = np.random.default_rng(seed)
     temp = rng.uniform(190.0, 310.0, ncell)
@@ -138,6 +165,8 @@ def box_cells(mech, ncell=4096, seed=0, dt=1200.0):
     photol = np.where(dark[None, :], 0.0, j0[:, None] * np.clip(cossza, 0, 1)[None, :] ** 0.8)
     logdev = rng.uniform(-1.0, 1.0, (mech.nspec, ncell))
     conc = _species_conc(mech, numden, logdev)
+    if mech.name == "carbon":
+        _carbon_inputs(mech, met, conc, rng)
     return Workload(mech, met, np.ascontiguousarray(photol), clear_sky_het(),
                     conc, np.zeros(ncell), dt, "box%d_seed%d" % (ncell, seed))
 
@@ -156,8 +185,9 @@ def _smooth_field(rng, shape, corr_cells):
     return f / (f.std() + 1e-300)
 
 
-def grid_cells(mech, nx=72, ny=46, nz=72, seed=1, dt=1200.0, nspec_fields=None):
-    """Cells of an nx*ny*nz lat/lon grid, ordered I fastest then J then L."""
+def grid_cells(mech, nx=72, ny=46, nz=72, seed=1, dt=1200.0, nspec_fields=None, hour_utc=12.0):
+    """Cells of an nx*ny*nz lat/lon grid, ordered I fastest then J then L;
+    the sun as at hour_utc on day 172."""
     rng = np.random.default_rng(seed)
     g = json.loads((_HERE.parent / "mech" / "grid72.json").read_text())
     ap, bp = np.asarray(g["ap_hpa"]), np.asarray(g["bp"])
@@ -178,7 +208,7 @@ def grid_cells(mech, nx=72, ny=46, nz=72, seed=1, dt=1200.0, nspec_fields=None):
     h2o = h2o_vmr * numden
     # solar zenith angle at 12:00 UTC, day 172 (solstice declination)
     decl = math.radians(23.44) * math.sin(2 * math.pi * (284 + 172) / 365.0)
-    hour_angle = np.radians(lon)                 # local solar time offset at 12 UTC
+    hour_angle = np.radians(lon + 15.0 * (hour_utc - 12.0))   # local solar hour angle
     latr = np.radians(lat)
     cossza2d = (np.sin(latr)[:, None] * math.sin(decl)
                 + np.cos(latr)[:, None] * math.cos(decl) * np.cos(hour_angle)[None, :])
@@ -198,8 +228,12 @@ def grid_cells(mech, nx=72, ny=46, nz=72, seed=1, dt=1200.0, nspec_fields=None):
     mix /= np.linalg.norm(mix, axis=1, keepdims=True)
     logdev = 0.5 * (mix @ base)
     conc = _species_conc(mech, flat(numden), logdev)
+    if mech.name == "carbon":
+        _carbon_inputs(mech, met, conc, rng)
     return Workload(mech, met, np.ascontiguousarray(photol), clear_sky_het(),
-                    conc, np.zeros(ncell), dt, "grid%dx%dx%d_seed%d" % (nx, ny, nz, seed))
+                    conc, np.zeros(ncell), dt,
+                    "grid%dx%dx%d_seed%d%s" % (nx, ny, nz, seed,
+                                               "" if hour_utc == 12.0 else "_%gUTC" % hour_utc))
 
 
 def rich_het(rng, ncell):

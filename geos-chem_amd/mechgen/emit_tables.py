@@ -78,10 +78,11 @@ def build_tables(m):
             pt_s += others + [-1] * (RX_MAX - 1 - len(others))
     # JVS gather in LU layout
     lu_irow, lu_icol = m["lu_irow"], m["lu_icol"]
-    nz = m["lu_nonzero"]
-    pos = {(lu_irow[k], lu_icol[k]): k for k in range(nz)} if lu_irow else {}
+    # a mechanism without a sparse Jacobian (carbon: forward Euler) has none
+    nz = m["lu_nonzero"] if lu_irow else 0
+    pos = {(lu_irow[k], lu_icol[k]): k for k in range(nz)}
     jterms = [[] for _ in range(nz)]
-    for ri, r in enumerate(rx):
+    for ri, r in enumerate(rx if nz else []):
         for s, k in r["reactants"]:
             if s >= nvar:
                 continue
@@ -173,5 +174,5 @@ def emit(name):
 
 
 if __name__ == "__main__":
-    for n in sys.argv[1:] or ["fullchem", "Hg"]:
+    for n in sys.argv[1:] or ["fullchem", "Hg", "carbon"]:
         emit(n)

@@ -44,6 +44,9 @@ AUTHORED_BG = {
 DEFAULT_BG = 1.0e-12      # any other stable species
 DEFAULT_BG_RADICAL = 1.0e-14  # names ending in O2/OO (peroxy radicals)
 DEFAULT_ATOL, DEFAULT_RTOL = 1.0e-2, 0.5e-2
+# mechanism-specific defaults for species without KPP_RelTol: Hg uses 1e-2
+# (GeosCore/mercury_mod.F90:3842-3843)
+DEFAULT_RTOL_BY_MECH = {"fullchem": 0.5e-2, "Hg": 1.0e-2, "carbon": 1.0e-2}
 
 
 def load_db(refroot):
@@ -51,7 +54,7 @@ def load_db(refroot):
     return yaml.load(path.read_text(), Loader=yaml.SafeLoader)
 
 
-def species_constants(db, names):
+def species_constants(db, names, mech="fullchem"):
     out = {k: [] for k in ("mw", "henry_k0", "henry_cr", "atol", "rtol", "bg_vv", "bg_src")}
     for n in names:
         e = db.get(n) or {}
@@ -59,7 +62,7 @@ def species_constants(db, names):
         out["henry_k0"].append(float(e.get("Henry_K0", 0.0) or 0.0))
         out["henry_cr"].append(float(e.get("Henry_CR", 0.0) or 0.0))
         out["atol"].append(float(e.get("KPP_AbsTol", DEFAULT_ATOL)))
-        out["rtol"].append(float(e.get("KPP_RelTol", DEFAULT_RTOL)))
+        out["rtol"].append(float(e.get("KPP_RelTol", DEFAULT_RTOL_BY_MECH.get(mech, DEFAULT_RTOL))))
         if n in AUTHORED_BG:
             out["bg_vv"].append(AUTHORED_BG[n])
             out["bg_src"].append("authored")
@@ -78,7 +81,7 @@ if __name__ == "__main__":
     here = Path(__file__).resolve().parent
     m = json.loads((here.parent / "mech" / ("%s.json" % mech)).read_text())
     db = load_db(root)
-    sc = species_constants(db, m["species"])
+    sc = species_constants(db, m["species"], mech)
     sc["species"] = m["species"]
     (here.parent / "mech" / ("%s_species.json" % mech)).write_text(json.dumps(sc))
     nz = sum(1 for v in sc["mw"] if v == 0.0)

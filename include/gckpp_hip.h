@@ -123,6 +123,118 @@ int gckpp_solve_dev(gckpp_ctx *ctx, int64_t ncell, const double *lu, double *x,
  * (GeosCore/fullchem_mod.F90:596-598,1298-1301). */
 int gckpp_step_timings(gckpp_ctx *ctx, float ms[4]);
 
+/* Persistent solver configuration (no reference counterpart; the OpenMP
+ * schedule of the cell loop, GeosCore/fullchem_mod.F90:528-546, is the CPU
+ * analogue).  variant in [0,5): loads kept in flight per sparse gather
+ * (1, 4, 8, 4, 8) x register budget (unconstrained for 0-2, at most 128
+ * VGPRs for 3-4); waves_per_group 1, 2 or 4 (cells per workgroup 64, 128,
+ * 256).  get: out = {variant, waves_per_group, waves per launch, loads in
+ * flight, minimum waves per SIMD}. */
+int gckpp_set_persistent_config(gckpp_ctx *ctx, int variant, int waves_per_group);
+int gckpp_get_persistent_config(gckpp_ctx *ctx, int32_t out[5]);
+
+/* Persistent solver extras (no reference counterpart).  order[ncell]
+ * (device, a permutation of 0..ncell-1, or null): the order in which the
+ * resident waves take cells, e.g. the costliest first.  hlog[2*n][ncell]
+ * (device float, or null): the next persistent Integrate or step writes
+ * (H, scaled error) of each cell's first n attempts to rows 2k, 2k+1 -- the
+ * step histories the step-size MLP is trained on.  The pointers are used by
+ * later calls until replaced. */
+int gckpp_set_cell_order(gckpp_ctx *ctx, const int32_t *order);
+int gckpp_set_step_log(gckpp_ctx *ctx, float *hlog, int nattempt);
+
+/* Autotuner: integrates a copy of the given cells with every persistent
+ * configuration, keeps the fastest for this context and writes the times
+ * [ms] to ms[variant*3 + log2(waves_per_group)] (-1: does not fit).  conc is
+ * not modified.  Synchronous on stream. */
+int gckpp_autotune_dev(gckpp_ctx *ctx, int64_t ncell, double tin, double tout,
+                       const int32_t icntrl[20], const double rcntrl[20],
+                       const double *hstart, const double *conc,
+                       const double *rconst, float *ms, void *stream);
+
+/* Mixed-mechanism batch (no reference counterpart: GEOS-Chem runs one
+ * mechanism per simulation, Do_FullChem / ChemMercury / Chem_Carbon_Gases,
+ * GeosCore/fullchem_mod.F90:528, mercury_mod.F90:646, carbon_gases_mod.F90:450).
+ * One chemistry step of the cells of up to three mechanisms is one
+ * persistent launch taking fullchem cells first, then Hg, then carbon:
+ * Update_RCONST of each part, Integrate (Rodas3; carbon: forward Euler), the
+ * callers' retry with Hstart=0 from the original C (fullchem and Hg), the
+ * clip C=max(C,0) (fullchem and Hg; carbon stores C as is).  A step can be
+ * captured into a HIP graph and replayed: each replay advances conc, and
+ * hstart (in: RCNTRL(3) per cell, out: RSTATUS(3)) carries the step size to
+ * the next step. */
+typedef struct gckpp_batch gckpp_batch;
+typedef struct {
+  int64_t ncell;              /* cells of this mechanism (0: none) */
+  const double *met, *photol, *het;
+  int64_t het_stride;         /* 0 or ncell */
+  double *conc;               /* [NSPEC][ncell] in/out */
+  double *hstart;             /* [ncell] in/out */
+  int32_t *istatus;           /* [20][ncell] or null */
+  int32_t *ierr;              /* [ncell] or null */
+  int32_t icntrl[20];
+  double rcntrl[20];
+  const int32_t *order;       /* [ncell] order in which cells are taken, or null */
+} gckpp_part;
+/* contexts in the order fullchem, Hg, carbon; any may be null; each part's
+ * ncell must not exceed its context's max_cells. */
+int gckpp_batch_init(gckpp_ctx *fullchem, gckpp_ctx *hg, gckpp_ctx *carbon,
+                     gckpp_batch **out);
+int gckpp_batch_set_config(gckpp_batch *b, int variant, int waves_per_group);
+int gckpp_batch_step_dev(gckpp_batch *b, const gckpp_part parts[3], double dt,
+                         void *stream);
+int gckpp_batch_capture(gckpp_batch *b, const gckpp_part parts[3], double dt);
+int gckpp_batch_replay(gckpp_batch *b, void *stream);
+int gckpp_batch_finalize(gckpp_batch *b);
+
+/* Step-size MLP (no reference counterpart; it replaces the cold-start
+ * Hstart of 1e-5 s, gckpp_Integrator.F90:389-395, by a prediction, and the
+ * integrator's step control still accepts or rejects every step).
+ * 40 inputs: T, log10 P, log10 NUMDEN, SUNCOS, log10(J+1e-20) of the 12
+ * PHOTOL slots jslot[] (0-based), log10(C+1) of the 24 species spc[]
+ * (0-based KPP indices); normalised by mean/std (48 entries, the last 8
+ * unused).  3 hidden layers of 128 GELU units; outputs (ln h0 - out_mu)/
+ * out_sigma and 4 stiffness-class logits.  Parameters: GCKPP_MLP_NPARAM
+ * floats, per layer W[in][out] row-major then bias, layer sizes 48x128,
+ * 128x128, 128x128, 128x16 (outputs 5..15 unused).
+ * predict: bf16 MFMA inference; h0[ncell] = exp(ln h0)*safety clipped to
+ *   [hmin, hmax], cls[ncell] = argmax class (either may be null).
+ * features: X[ncell][48] fp32 normalised inputs (for training).
+ * forward: fp32 outputs out[nb][16] of X[nb][48].
+ * grad / train_step: fp32 forward and backward of the loss
+ *   wh*mean((o0 - yh)^2) + wc*mean(cross-entropy(o1..o4, ycls)) over nb
+ *   rows; grad[GCKPP_MLP_NPARAM] (device) and loss[1] (device) may be null;
+ *   train_step then applies one Adam step (b1 0.9, b2 0.999, eps 1e-8). */
+#define GCKPP_MLP_NPARAM 41360
+typedef struct gckpp_mlp gckpp_mlp;
+int gckpp_mlp_init(int64_t max_batch, const int32_t jslot[12], const int32_t spc[24],
+                   gckpp_mlp **out);
+int gckpp_mlp_set_norm(gckpp_mlp *m, const float mean[48], const float stdev[48],
+                       float out_mu, float out_sigma);
+int gckpp_mlp_set_params(gckpp_mlp *m, const float *params);
+int gckpp_mlp_get_params(gckpp_mlp *m, float *params);
+int gckpp_mlp_features_dev(gckpp_mlp *m, int64_t ncell, const double *met,
+                           const double *photol, const double *conc, float *X,
+                           void *stream);
+int gckpp_mlp_predict_dev(gckpp_mlp *m, int64_t ncell, const double *met,
+                          const double *photol, const double *conc, double safety,
+                          double hmin, double hmax, double *h0, int32_t *cls,
+                          void *stream);
+int gckpp_mlp_forward_dev(gckpp_mlp *m, int64_t nb, const float *X, float *out,
+                          void *stream);
+int gckpp_mlp_grad_dev(gckpp_mlp *m, int64_t nb, const float *X, const float *yh,
+                       const int32_t *ycls, float wh, float wc, float *grad,
+                       float *loss, void *stream);
+int gckpp_mlp_train_step_dev(gckpp_mlp *m, int64_t nb, const float *X, const float *yh,
+                             const int32_t *ycls, float wh, float wc, float lr,
+                             float *loss, void *stream);
+int gckpp_mlp_finalize(gckpp_mlp *m);
+/* order[ncell] = the cells sorted by class, class 3 (costliest) first,
+ * stable within a class: a queue order for gckpp_set_cell_order.
+ * scratch: 4*ceil(ncell/256) int32 on the device. */
+int gckpp_order_by_class_dev(int64_t ncell, const int32_t *cls, int32_t *order,
+                             int32_t *scratch, void *stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *gckpp_last_error(void);
 

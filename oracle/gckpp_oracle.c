@@ -318,3 +318,31 @@ void orc_integrate(const orc_mech *m, int64_t ncell, double *conc,
                    tout, atol, rtol, icntrl, rcntrl, istatus + c * 20,
                    rstatus + c * 20, ierr + c);
 }
+
+/* The carbon mechanism's Integrate: one forward-Euler step
+ * (KPP/carbon/gckpp_Integrator.F90:93-150 and ForwardEuler :155-215).
+ * The reference's Fun there is the aggregate form (A(r) then Vdot = sum of
+ * stoichiometric terms), which orc_fun equals up to summation order. */
+void orc_euler(const orc_mech *m, int64_t ncell, double *conc, const double *rct,
+               double tin, double tout, const int *icntrl, int *ierr) {
+  int nspec = m->nvar + m->nfix;
+  int icu16 = icntrl ? icntrl[15] : 0;
+  double *Ynew = malloc(sizeof(double) * m->nvar);
+  double *dYdt = malloc(sizeof(double) * m->nvar);
+  for (int64_t c = 0; c < ncell; ++c) {
+    double *Y = conc + c * nspec;
+    orc_fun(m, Y, Y + m->nvar, rct + c * m->nreact, dYdt);
+    for (int i = 0; i < m->nvar; ++i) Ynew[i] = Y[i] + dYdt[i] * (tout - tin);
+    int e = 1, stop = 0;
+    if (icu16 > 0)
+      for (int i = 0; i < m->nvar && !stop; ++i)
+        if (Ynew[i] < 0.0) {
+          if (icu16 == 1) Ynew[i] = 0.0;
+          else if (icu16 == 2 || icu16 == 3) { e = -9; stop = 1; }
+        }
+    if (!stop) memcpy(Y, Ynew, sizeof(double) * m->nvar);
+    ierr[c] = e;
+  }
+  free(Ynew);
+  free(dYdt);
+}

@@ -43,3 +43,9 @@ void orc_integrate(const orc_mech *m, int64_t ncell, double *conc,
                    double tout, const double *atol, const double *rtol,
                    const int *icntrl, const double *rcntrl, int *istatus,
                    double *rstatus, int *ierr, int nthreads);
+
+/* The carbon mechanism's Integrate: Y += Fun(Y)*(tout-tin), ICNTRL(16)
+ * negative handling (KPP/carbon/gckpp_Integrator.F90:155-215).  conc
+ * [ncell][nspec] in/out, rct [ncell][nreact], ierr [ncell]. */
+void orc_euler(const orc_mech *m, int64_t ncell, double *conc, const double *rct,
+               double tin, double tout, const int *icntrl, int *ierr);

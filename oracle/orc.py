@@ -65,6 +65,8 @@ class Oracle:
         L.orc_solve.argtypes = [mp, _D, _D]
         L.orc_integrate.argtypes = [mp, ctypes.c_int64, _D, _D, _D, ctypes.c_double,
                                     ctypes.c_double, _D, _D, _I, _D, _I, _D, _I, ctypes.c_int]
+        L.orc_euler.argtypes = [mp, ctypes.c_int64, _D, _D, ctypes.c_double, ctypes.c_double,
+                                _I, _I]
 
     def _p(self):
         return ctypes.byref(self.ms)
@@ -127,3 +129,14 @@ class Oracle:
                                rc.ctypes.data_as(_D), ist.ctypes.data_as(_I),
                                rst.ctypes.data_as(_D), ierr.ctypes.data_as(_I), int(nthreads))
         return np.ascontiguousarray(c.T), np.ascontiguousarray(ist.T), np.ascontiguousarray(rst.T), ierr
+
+    def euler(self, conc, rct, tin, tout, icntrl):
+        """The carbon mechanism's forward-Euler Integrate; returns (conc, ierr)."""
+        n = conc.shape[1]
+        c = np.ascontiguousarray(conc.T, dtype=np.float64).copy()
+        r = np.ascontiguousarray(rct.T, dtype=np.float64)
+        ierr = np.zeros(n, dtype=np.int32)
+        ic = _i32(icntrl)
+        self.lib.orc_euler(self._p(), n, c.ctypes.data_as(_D), r.ctypes.data_as(_D), tin, tout,
+                           ic.ctypes.data_as(_I), ierr.ctypes.data_as(_I))
+        return np.ascontiguousarray(c.T), ierr

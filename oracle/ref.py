@@ -135,3 +135,23 @@ def fullchem_icntrl():
     ic[0], ic[2], ic[6], ic[14] = 1, 4, 1, -1
     rc = np.zeros(20)
     return ic, rc
+
+
+def hg_icntrl():
+    """ICNTRL/RCNTRL of the Hg chemistry call (GeosCore/mercury_mod.F90:951-969):
+    autonomous, vector tolerances, Rodas3, no adjoint, rates frozen."""
+    ic = np.zeros(20, dtype=np.int32)
+    ic[0], ic[2], ic[6], ic[14] = 1, 4, 1, -1
+    return ic, np.zeros(20)
+
+
+def carbon_icntrl():
+    """ICNTRL of the carbon chemistry call (GeosCore/carbon_gases_mod.F90:454-457):
+    verbose errors, keep negatives, rates frozen."""
+    ic = np.zeros(20, dtype=np.int32)
+    ic[0], ic[14] = 1, -1
+    return ic, np.zeros(20)
+
+
+def icntrl_for(mech_name):
+    return {"fullchem": fullchem_icntrl, "Hg": hg_icntrl, "carbon": carbon_icntrl}[mech_name]()

@@ -17,9 +17,11 @@ MODULE gckpp_ref_driver
   USE gckpp_Parameters
   USE gckpp_Global
   USE gckpp_Rates,         ONLY : Update_RCONST
-  USE gckpp_Function,      ONLY : Fun_SPLIT
+  USE gckpp_Function,      ONLY : Fun, Fun_SPLIT
+#ifndef MECH_CARBON
   USE gckpp_Jacobian,      ONLY : Jac_SP
   USE gckpp_LinearAlgebra, ONLY : KppDecomp, KppSolve
+#endif
   USE gckpp_Integrator,    ONLY : Integrate
   IMPLICIT NONE
   PRIVATE
@@ -117,10 +119,16 @@ CONTAINS
     REAL(C_DOUBLE),     INTENT(OUT):: vdot(ncell,NVAR)
     REAL(dp) :: V(NVAR), F(NFIX), RCT(NREACT), Vd(NVAR), P(NVAR), D(NVAR)
     INTEGER(C_INT64_T) :: i
+#if !defined(MECH_HG) && !defined(MECH_CARBON)
     DO_FUN = .TRUE.
+#endif
     DO i = 1, ncell
        V = conc(i,1:NVAR); F = conc(i,NVAR+1:NSPEC); RCT = rct_io(i,:)
+#ifdef MECH_CARBON
+       CALL Fun( V, F, RCT, Vd )
+#else
        CALL Fun_SPLIT( V, F, RCT, Vd, P, D )
+#endif
        vdot(i,:) = Vd
     ENDDO
   END SUBROUTINE ref_fun
@@ -132,13 +140,17 @@ CONTAINS
     REAL(C_DOUBLE),     INTENT(OUT):: jvs(ncell,LU_NONZERO)
     REAL(dp) :: V(NVAR), F(NFIX), RCT(NREACT), J(LU_NONZERO)
     INTEGER(C_INT64_T) :: i
+#if !defined(MECH_HG) && !defined(MECH_CARBON)
     DO_JVS = .TRUE.
+#endif
+#ifndef MECH_CARBON
     DO i = 1, ncell
        V = conc(i,1:NVAR); F = conc(i,NVAR+1:NSPEC); RCT = rct_io(i,:)
        J = 0.0_dp
        CALL Jac_SP( V, F, RCT, J )
        jvs(i,:) = J
     ENDDO
+#endif
   END SUBROUTINE ref_jac
 
   ! KppDecomp in place (gckpp_LinearAlgebra.F90:46-83); ier(i) = IER.
@@ -149,12 +161,14 @@ CONTAINS
     REAL(dp) :: J(LU_NONZERO)
     INTEGER  :: e
     INTEGER(C_INT64_T) :: i
+#ifndef MECH_CARBON
     DO i = 1, ncell
        J = jvs(i,:)
        CALL KppDecomp( J, e )
        jvs(i,:) = J
        ier(i) = e
     ENDDO
+#endif
   END SUBROUTINE ref_decomp
 
   ! KppSolve in place (gckpp_LinearAlgebra.F90:644-2309).
@@ -164,12 +178,16 @@ CONTAINS
     REAL(C_DOUBLE),     INTENT(INOUT) :: x(ncell,NVAR)
     REAL(dp) :: J(LU_NONZERO), B(NVAR)
     INTEGER(C_INT64_T) :: i
+#if !defined(MECH_HG) && !defined(MECH_CARBON)
     DO_SLV = .TRUE.
+#endif
+#ifndef MECH_CARBON
     DO i = 1, ncell
        J = lu(i,:); B = x(i,:)
        CALL KppSolve( J, B )
        x(i,:) = B
     ENDDO
+#endif
   END SUBROUTINE ref_solve
 
   ! Integrate(TIN,TOUT,...) per cell with rates given (ICNTRL(15)=-1 keeps
@@ -244,7 +262,12 @@ CONTAINS
           CALL Integrate( 0.0_dp, dtchem, IC, RC, IS, RS, IE )
        ENDIF
        hnew(i) = RS(3)
+#ifdef MECH_CARBON
+       ! carbon_ConvertMolecCm3ToKg stores C as it is (carbon_Funcs.F90:300-322)
+       conc(i,:) = C(1:NSPEC)
+#else
        conc(i,:) = MAX( C(1:NSPEC), 0.0_dp )
+#endif
        istatus(i,:) = IS
        ierr(i) = IE
     ENDDO

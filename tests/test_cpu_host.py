@@ -35,6 +35,7 @@ def test_mech_sizes_without_gpu():
         pytest.skip("HIP library not built")
     assert hip.mech_sizes(0) == [356, 353, 3, 1058, 5683, fields.NMET, fields.NHET, 200]
     assert hip.mech_sizes(1)[:5] == [43, 32, 11, 94, 161]
+    assert hip.mech_sizes(2)[:5] == [17, 12, 5, 8, 1]   # KPP/carbon/gckpp_Parameters.F90
     with pytest.raises(hip.GckppError):
         hip.mech_sizes(7)
 

@@ -1,0 +1,199 @@
+"""GPU tests of the step-size MLP kernels (csrc/mlp.hip) through the C ABI,
+against the numpy restatement oracle/mlp.py (itself pinned by finite
+differences on the CPU), and of what the trained network gains.
+
+Tolerances: fp32 kernels against float64 numpy: features and forward
+outputs 1e-4 relative (plus 1e-5 absolute), gradients 2e-3 of each layer's
+largest entry (fp32 sums over 1,024 rows); bf16 inference against the
+numpy bf16 emulation of the same arithmetic: 5e-3 in the normalised output
+(the kernel sums in fp32, the emulation in float64)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _t(torch, a, dtype=None):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype=dtype or torch.float64)
+
+
+@pytest.fixture(scope="module")
+def setup(fullchem, torch_dev):
+    from gckpp_amd import hip, learned, synth
+    w = synth.box_cells(fullchem, 2048, 3)
+    jslot, spc = learned.inputs(fullchem)
+    rng = np.random.default_rng(1)
+    mean = np.zeros(48, np.float32)
+    std = np.ones(48, np.float32)
+    mean[:40] = rng.normal(0, 2, 40)
+    std[:40] = rng.uniform(0.5, 3.0, 40)
+    net = hip.MLP(4096, jslot, spc)
+    net.set_norm(mean, std, -2.0, 1.7)
+    p = (rng.normal(0.0, 0.12, hip.MLP_NPARAM)).astype(np.float32)
+    net.set_params(p)
+    yield dict(w=w, jslot=jslot, spc=spc, mean=mean, std=std, net=net, p=p)
+    net.close()
+
+
+def test_features_match_numpy(setup, torch_dev):
+    print("case: MLP features, 2048 box cells", flush=True)
+    from oracle import mlp
+    s, w = setup, setup["w"]
+    X = s["net"].features(_t(torch_dev, w.met), _t(torch_dev, w.photol), _t(torch_dev, w.conc))
+    ref = mlp.features(s["jslot"], s["spc"], s["mean"].astype(np.float64), s["std"].astype(np.float64),
+                       w.met, w.photol, w.conc)
+    x = X.cpu().numpy()
+    assert np.all(np.abs(x - ref) <= 1e-4 * np.abs(ref) + 1e-5)
+    assert np.all(x[:, 40:] == 0.0)
+
+
+def test_forward_and_gradient_match_numpy(setup, torch_dev):
+    print("case: MLP fp32 forward and gradient, 1024 rows", flush=True)
+    from oracle import mlp
+    rng = np.random.default_rng(2)
+    X = rng.normal(size=(1024, 48)).astype(np.float32)
+    X[:, 40:] = 0
+    yh = rng.normal(size=1024).astype(np.float32)
+    yc = rng.integers(0, 4, 1024).astype(np.int32)
+    net, p = setup["net"], setup["p"]
+    out = net.forward(_t(torch_dev, X, torch_dev.float32)).cpu().numpy()
+    _, _, o_ref = mlp.forward(p.astype(np.float64), X.astype(np.float64))
+    assert np.all(np.abs(out - o_ref) <= 1e-4 * np.abs(o_ref) + 1e-5)
+    g, loss = net.grad(_t(torch_dev, X, torch_dev.float32), _t(torch_dev, yh, torch_dev.float32),
+                       _t(torch_dev, yc, torch_dev.int32), wh=0.7, wc=1.3)
+    g = g.cpu().numpy()
+    L_ref, g_ref = mlp.loss_grad(p.astype(np.float64), X.astype(np.float64), yh.astype(np.float64),
+                                 yc, 0.7, 1.3)
+    assert abs(loss.item() - L_ref) <= 1e-4 * abs(L_ref)
+    for (w0, b0), (a, b) in zip(mlp.OFFS, mlp.SHAPES):
+        for lo, hi in ((w0, w0 + a * b), (b0, b0 + b)):
+            scale = np.abs(g_ref[lo:hi]).max()
+            assert np.abs(g[lo:hi] - g_ref[lo:hi]).max() <= 2e-3 * scale, (lo, hi)
+
+
+def test_bf16_mfma_inference_matches_emulation(setup, torch_dev):
+    print("case: MLP bf16 MFMA predict, 2048 box cells", flush=True)
+    from oracle import mlp
+    s, w = setup, setup["w"]
+    h0, cls = s["net"].predict(_t(torch_dev, w.met), _t(torch_dev, w.photol), _t(torch_dev, w.conc),
+                               safety=1.0, hmin=1e-300, hmax=1e300)
+    X = mlp.features(s["jslot"], s["spc"], s["mean"].astype(np.float64), s["std"].astype(np.float64),
+                     w.met, w.photol, w.conc).astype(np.float32)
+    o = mlp.forward_bf16(s["p"], X)
+    lnh = np.log(h0.cpu().numpy())
+    out0 = (lnh - (-2.0)) / 1.7
+    assert np.abs(out0 - o[:, 0]).max() <= 5e-3 * max(1.0, np.abs(o[:, 0]).max())
+    # classes agree wherever the emulated top-2 logits are not within roundoff
+    lg = o[:, 1:5]
+    top = np.sort(lg, axis=1)
+    clear = top[:, -1] - top[:, -2] > 1e-2
+    assert np.array_equal(cls.cpu().numpy()[clear], lg.argmax(axis=1)[clear])
+    # and the fp32 network agrees with bf16 inference to bf16 precision
+    _, _, o32 = mlp.forward(s["p"].astype(np.float64), X.astype(np.float64))
+    assert np.abs(o32[:, 0] - o[:, 0]).max() <= 0.05 * max(1.0, np.abs(o32[:, 0]).max())
+
+
+def test_order_by_class(torch_dev):
+    print("case: order by class, 100000 cells", flush=True)
+    from gckpp_amd import hip
+    rng = np.random.default_rng(4)
+    c = rng.integers(0, 4, 100_000).astype(np.int32)
+    order = hip.order_by_class(_t(torch_dev, c, torch_dev.int32)).cpu().numpy()
+    ref = np.argsort(3 - c, kind="stable")
+    assert np.array_equal(order, ref)
+
+
+def test_training_reduces_loss(setup, torch_dev):
+    print("case: MLP training on a synthetic target", flush=True)
+    from gckpp_amd import hip
+    T = torch_dev
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(32768, 48)).astype(np.float32)
+    X[:, 40:] = 0
+    yh = (np.tanh(X[:, 0] - 0.5 * X[:, 3]) + 0.3 * X[:, 7]).astype(np.float32)
+    yc = (np.digitize(X[:, 1] + X[:, 2], [-1.0, 0.0, 1.0])).astype(np.int32)
+    net = hip.MLP(4096, setup["jslot"], setup["spc"])
+    p = (rng.normal(0, 1, hip.MLP_NPARAM) * 0.08).astype(np.float32)
+    net.set_params(p)
+    Xd, yd, cd = _t(T, X, T.float32), _t(T, yh, T.float32), _t(T, yc, T.int32)
+    first = None
+    for ep in range(6):
+        for s in range(0, 32768, 4096):
+            loss = net.train_step(Xd[s:s + 4096], yd[s:s + 4096], cd[s:s + 4096], 2e-3)
+            if first is None:
+                first = loss.item()
+    last = loss.item()
+    print("loss", first, "->", last)
+    assert last < 0.2 * first
+    net.close()
+
+
+def test_learned_h0_saves_attempts_within_tolerance(fullchem, torch_dev):
+    """Trained on the step histories the GPU solver logs on two synthetic
+    grids, the network's h0 and cost order cut the attempts per cell on a
+    third (held-out) grid, and every cell stays within the repo tolerance of
+    the cold-start solve."""
+    print("case: learned h0 gain, 36x23x24 grids", flush=True)
+    from gckpp_amd import hip, learned, synth
+    from oracle.ref import fullchem_icntrl
+    T = torch_dev
+    ic, rc = fullchem_icntrl()
+    ctx = hip.Context(fullchem, 36 * 23 * 24)
+    model = learned.StepModel(fullchem, max_batch=8192)
+
+    def dev(w):
+        return dict(met=_t(T, w.met), photol=_t(T, w.photol), het=_t(T, w.het), conc=_t(T, w.conc))
+
+    Xs, lh, ns = [], [], []
+    for seed, hour in ((21, 6.0), (22, 15.0)):
+        w = synth.grid_cells(fullchem, 36, 23, 24, seed=seed, hour_utc=hour)
+        d = dev(w)
+        hlog, nstep, ierr = learned.log_step_histories(ctx, d, ic, rc, w.dt)
+        assert (ierr == 1).all()
+        Xs.append(model.raw_features(d["met"], d["photol"], d["conc"]))
+        lh.append(learned.labels(hlog, nstep, w.dt))
+        ns.append(nstep)
+    Xraw = T.cat(Xs)
+    lnh, nstep = np.concatenate(lh), np.concatenate(ns)
+    model.fit_normalisation(Xraw, lnh, nstep)
+    X = model.normalise(Xraw)
+    yh = _t(T, (lnh - model.out_mu) / model.out_sigma, T.float32)
+    yc = _t(T, learned.classes(nstep, model.thresholds), T.int32)
+    hist = model.train(X, yh, yc, epochs=6, batch=2048, lr=3e-3)
+    print("training loss per epoch", hist)
+    assert hist[-1] < hist[0]
+    # held-out grid
+    w = synth.grid_cells(fullchem, 36, 23, 24, seed=23, hour_utc=10.0)
+    d = dev(w)
+    n = w.ncell
+    res = {}
+    for label in ("cold", "learned"):
+        ctx.set_solver(hip.SOLVER_PERSISTENT)
+        conc = d["conc"].clone()
+        if label == "cold":
+            hs = T.zeros(n, dtype=T.float64, device="cuda")
+            ctx.set_cell_order(None)
+        else:
+            hs, cls = model.predict(d["met"], d["photol"], d["conc"], safety=0.5, dt=w.dt)
+            ctx.set_cell_order(hip.order_by_class(cls))
+        hnew, ist, ierr, ok = ctx.fullchem_step(d["met"], d["photol"], d["het"], conc, hs, w.dt, ic, rc)
+        assert ok
+        res[label] = (conc.cpu().numpy(), ist.cpu().numpy())
+    ctx.set_cell_order(None)
+    a_cold, a_learn = res["cold"][1][2].sum(), res["learned"][1][2].sum()
+    print("attempts: cold %d learned %d (%.1f%% fewer)" % (a_cold, a_learn, 100 * (1 - a_learn / a_cold)))
+    assert a_learn < 0.9 * a_cold
+    nv = fullchem.nvar
+    c0, c1 = res["cold"][0], res["learned"][0]
+    err = np.abs(c1[:nv] - c0[:nv]) / (fullchem.atol()[:, None] + fullchem.rtol()[:, None] * np.abs(c0[:nv]))
+    print("max scaled difference learned vs cold", err.max())
+    assert err.max() <= 1.0
+    ctx.close()
