@@ -1,13 +1,26 @@
-"""Benchmark: fullchem cells/s per 1200 s chemistry step on one MI355X.
+"""Benchmark: fullchem cells/s per 1200 s chemistry step on one MI355X,
+plain GPU Rosenbrock against the learned-h0 + persistent solver.
 
-Workload: the 4x5-degree 72x46x72 grid (238,464 cells) of synthetic inputs
-(gckpp_amd.synth.grid_cells, seed 1), cold start (Hstart = 1e-5 s), dt =
-1200 s, repo RTOL/ATOL.  A step is one Do_FullChem pass over every cell
-through the C ABI: Update_RCONST, Integrate (Rodas3), the retry of failed
-cells and the clipping of negatives.  Inputs are resident in HBM before the
-timed region; each step starts from the same initial concentrations.
+Default workload: the 4x5-degree 72x46x72 grid (238,464 cells) of synthetic
+inputs (gckpp_amd.synth.grid_cells, seed 1, 12 UTC), cold start, dt = 1200 s,
+repo RTOL/ATOL.  A step is one Do_FullChem pass over every cell through the
+C ABI: Update_RCONST, Integrate (Rodas3), the retry of failed cells and the
+clipping of negatives.  Inputs are resident in HBM before the timed region;
+each step starts from the same initial concentrations.
 
-Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+Two legs on the same inputs:
+  plain    one lane per cell (k_integrate), Hstart = 1e-5 s (cold start)
+  learned  the MLP predicts each cell's h0 and cost class (bf16 MFMA), the
+           cells are ordered costliest first, and the persistent solver
+           (k_persist, autotuned) integrates them; prediction and ordering
+           are inside the timed region.
+value = learned-leg cells/s; the plain leg is reported beside it.
+
+Other workloads (--workload): mixed (144x91x72, fullchem/Hg/carbon
+70/20/10, one persistent launch per step, 24 steps replayed from a HIP
+graph), c90 (6x90x90x72 = 3,499,200 cells in six face-sized chunks).
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -21,12 +34,10 @@ sys.path.insert(0, str(ROOT / "geos-chem_amd"))
 sys.path.insert(0, str(ROOT))
 
 METRIC = "fullchem cells/sec per 1200 s chem step at repo RTOL: plain GPU vs learned-h0"
-PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix) peak
-PEAK_HBM_GBS = 8000.0
-# FP64 work per Rodas3 attempt of fullchem, counted from the generated code:
-# first attempt of a step 145.7 kFLOP, a rejected re-attempt 122.3 kFLOP,
-# Update_RCONST ~5 kFLOP per cell.
-FLOP_ACC, FLOP_REJ, FLOP_RATES = 145.7e3, 122.3e3, 5.0e3
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 peak (vector and matrix)
+# FP64 work per Rodas3 attempt of fullchem, counted from the mechanism tables
+# (first attempt of a step 145.7 kFLOP, a rejected re-attempt 122.3 kFLOP).
+FLOP_ACC, FLOP_REJ = 145.7e3, 122.3e3
 
 
 def parse():
@@ -34,9 +45,13 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="grid", choices=["grid", "mixed", "c90"])
     p.add_argument("--grid", default="72x46x72", help="NXxNYxNZ of the synthetic grid")
     p.add_argument("--cpu-sample", type=int, default=8192,
                    help="cells of the same grid timed with the CPU reference (0: skip)")
+    p.add_argument("--tight-sample", type=int, default=48,
+                   help="cells checked against the RTOL 1e-8 oracle (0: skip)")
+    p.add_argument("--safety", type=float, default=0.5, help="factor on the predicted h0")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device copy (reference for HBM byte counters)")
     return p.parse_args()
@@ -55,26 +70,344 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": 1 << 30, "repeats": 5}), flush=True)
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def cpu_baseline(mech, w, ic, rc, nsample):
-    """The compiled reference (oracle/_ref) on a strided sample of the
-    workload, all host cores, Do_FullChem semantics (rates, Integrate, retry)."""
+    """The compiled reference (oracle/_ref) on an evenly strided sample of the
+    workload, all host cores, Do_FullChem semantics (rates, Integrate, retry,
+    clip).  Returns (baseline dict, sample indices, reference outputs)."""
     import numpy as np
     from oracle import ref
     if nsample <= 0 or not ref.available():
-        return None
+        return None, None, None
     R = ref.RefKPP(mech)
     idx = np.linspace(0, w.ncell - 1, nsample).astype(np.int64)
     ws = w.subset(idx)
     cores = len(os.sched_getaffinity(0))
     R.fullchem_step(ws.subset(np.arange(min(64, nsample))), ic, rc, nthreads=cores)  # warm-up
     t0 = time.perf_counter()
-    R.fullchem_step(ws, ic, rc, nthreads=cores)
+    c_ref, ist_ref, _, e_ref = R.fullchem_step(ws, ic, rc, nthreads=cores)
     dt = time.perf_counter() - t0
-    return {"value": round(nsample / dt, 2), "unit": "cells/s", "cores": cores,
-            "kind": "reference",
-            "sample": "%d cells spread evenly over the same %s grid (every %dth cell), "
-                      "reference KPP fullchem core compiled with flang, OpenMP "
-                      "DYNAMIC,24 over cells, one chem step" % (nsample, w.name, max(1, w.ncell // nsample))}
+    base = {"value": round(nsample / dt, 2), "unit": "cells/s", "cores": cores, "kind": "reference",
+            "sample": "%d cells spread evenly over the same %s grid (every %dth cell), reference "
+                      "KPP fullchem core compiled with flang, OpenMP DYNAMIC,24 over cells, one "
+                      "chem step (%.1f s)" % (nsample, w.name, max(1, w.ncell // nsample), dt)}
+    return base, idx, (c_ref, ist_ref, e_ref)
+
+
+def rel_errors(m, c, ref):
+    """Scaled error (|c-ref| / (ATOL + RTOL|ref|)) max, and per-species
+    relative error over entries above ATOL: max and median."""
+    import numpy as np
+    nv = m.nvar
+    a, r = m.atol()[:, None], m.rtol()[:, None]
+    sc = np.abs(c[:nv] - ref[:nv]) / (a + r * np.abs(ref[:nv]))
+    big = np.abs(ref[:nv]) > a
+    rel = np.abs(c[:nv] - ref[:nv])[big] / np.abs(ref[:nv])[big]
+    return {"max_scaled_err": float(sc.max()), "max_rel_err": float(rel.max()) if rel.size else 0.0,
+            "median_rel_err": float(np.median(rel)) if rel.size else 0.0,
+            "frac_within_rtol": float(np.mean(sc <= 1.0))}
+
+
+def tight_check(mech, w, idx_cells, gpu_c, ref_c, ic, rc):
+    """Integrate(0,dt) of a few cells with the C oracle at RTOL 1e-8, ATOL
+    1e-6 molec/cm3 (prod/loss dummies excluded): the errors of the GPU result
+    and of the reference against that tight solution (the step's clip
+    applied to it).  Errors are measured in the repo tolerance's units."""
+    import numpy as np
+    from oracle import orc, ref
+    ws = w.subset(idx_cells)
+    rct = ref.RefKPP(mech).rates(ws)
+    O = orc.Oracle(mech)
+    atol = np.full(mech.nvar, 1e-6)
+    atol[mech.atol() >= 1e20] = 1e25
+    rtol = np.full(mech.nvar, 1e-8)
+    c_t, _, _, e_t = O.integrate(ws.conc, rct, np.zeros(ws.ncell), 0.0, ws.dt, ic, rc, atol=atol,
+                                 rtol=rtol, nthreads=len(os.sched_getaffinity(0)))
+    ok = e_t == 1
+    c_t = np.maximum(c_t, 0.0)
+    out = {"cells": int(ok.sum()), "rtol": 1e-8}
+    out["gpu"] = rel_errors(mech, gpu_c[:, ok], c_t[:, ok])
+    if ref_c is not None:
+        out["reference"] = rel_errors(mech, ref_c[:, ok], c_t[:, ok])
+    return out
+
+
+def grid_bench(args):
+    import numpy as np
+    import torch
+    from gckpp_amd import hip, learned, mechanism, synth
+    from oracle.ref import fullchem_icntrl
+
+    mech = mechanism.load("fullchem")
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    t0 = time.time()
+    w = synth.grid_cells(mech, nx, ny, nz, seed=1)
+    n = w.ncell
+    log("workload %s: %d cells (%.1f s)" % (w.name, n, time.time() - t0))
+    ic, rc = fullchem_icntrl()
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    met, photol, het, conc0 = dev(w.met), dev(w.photol), dev(w.het), dev(w.conc)
+    cold = torch.zeros(n, dtype=torch.float64, device="cuda")
+    conc = conc0.clone()
+    hnew = torch.empty(n, dtype=torch.float64, device="cuda")
+    ist = torch.empty((20, n), dtype=torch.int32, device="cuda")
+    ierr = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx = hip.Context(mech, n)
+
+    model = None
+    mpath = learned.default_model_path("fullchem")
+    if mpath.exists():
+        model = learned.StepModel(mech).load(mpath)
+    else:
+        log("no trained model at %s: the learned leg runs with cold Hstart" % mpath)
+
+    # autotune the persistent solver on every 7th cell
+    sub = torch.arange(0, n, 7, device="cuda")
+    rct = ctx.rates(met, photol, het, conc0)
+    tune_ms = ctx.autotune(conc0[:, sub].contiguous(), rct[:, sub].contiguous(), cold[sub].contiguous(),
+                           0.0, w.dt, ic, rc)
+    del rct
+    pcfg = ctx.persistent_config()
+    log("autotuned persistent config %s" % pcfg)
+
+    h0 = torch.empty(n, dtype=torch.float64, device="cuda")
+    cls = torch.empty(n, dtype=torch.int32, device="cuda")
+
+    def step_plain():
+        conc.copy_(conc0)
+        ctx.set_solver(hip.SOLVER_PLAIN)
+        return ctx.fullchem_step(met, photol, het, conc, cold, w.dt, ic, rc, hnew=hnew, istatus=ist,
+                                 ierr=ierr)
+
+    def step_learned():
+        conc.copy_(conc0)
+        ctx.set_solver(hip.SOLVER_PERSISTENT)
+        if model is not None:
+            model.predict(met, photol, conc, safety=args.safety, dt=w.dt, h0=h0, cls=cls)
+            ctx.set_cell_order(hip.order_by_class(cls))
+            hs = h0
+        else:
+            ctx.set_cell_order(None)
+            hs = cold
+        r = ctx.fullchem_step(met, photol, het, conc, hs, w.dt, ic, rc, hnew=hnew, istatus=ist,
+                              ierr=ierr)
+        return r
+
+    legs = {}
+    for name, fn in (("plain", step_plain), ("learned", step_learned)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        tim = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+            tim.append(ctx.step_timings())
+            log("%s step done: %.1f ms total, integrate %.1f ms" % (name, tim[-1][3], tim[-1][1]))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = ist.cpu().numpy()
+        legs[name] = dict(ms=wall / args.steps * 1e3, cps=n / (wall / args.steps), tim=tim,
+                          conc=conc.cpu().numpy(), st=st, ierr=ierr.cpu().numpy())
+    ctx.set_cell_order(None)
+    L, P = legs["learned"], legs["plain"]
+
+    # dominant kernel: the persistent integrate of the learned leg
+    nacc, nstp = L["st"][3].astype(np.int64), L["st"][2].astype(np.int64)
+    flop = FLOP_ACC * nacc.sum() + FLOP_REJ * (nstp - nacc).sum()
+    k_ms = float(np.mean([t[1] for t in L["tim"]]))
+    achieved = flop / (k_ms * 1e-3) / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / "persist_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+
+    base, sidx, refout = cpu_baseline(mech, w, ic, rc, args.cpu_sample)
+    accuracy = {"learned_vs_plain_gpu": rel_errors(mech, L["conc"], P["conc"])}
+    if refout is not None:
+        accuracy["learned_vs_reference"] = rel_errors(mech, L["conc"][:, sidx], refout[0])
+        accuracy["plain_vs_reference"] = rel_errors(mech, P["conc"][:, sidx], refout[0])
+    if args.tight_sample > 0 and sidx is not None:
+        pick = np.linspace(0, len(sidx) - 1, args.tight_sample).astype(np.int64)
+        cells = sidx[pick]
+        accuracy["vs_rtol_1e-8_oracle"] = tight_check(mech, w, cells, L["conc"][:, cells],
+                                                      refout[0][:, pick], ic, rc)
+
+    out = {
+        "metric": METRIC,
+        "value": round(L["cps"], 2),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(L["ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "fullchem chemistry step, %dx%dx%d synthetic grid (seed 1, 12 UTC), "
+                               "%d cells, dt=1200 s, cold start, repo RTOL/ATOL; learned h0 (bf16 "
+                               "MFMA MLP) + cost-ordered persistent Rodas3" % (nx, ny, nz, n),
+                   "cells": n, "learned_model": model is not None, "h0_safety": args.safety,
+                   "persistent_config": pcfg},
+        "plain_gpu": {"value": round(P["cps"], 2), "unit": "cells/s", "ms_per_step": round(P["ms"], 3),
+                      "solver": "one lane per cell (k_integrate), Hstart 1e-5 s"},
+        "speedup_vs_plain": round(L["cps"] / P["cps"], 3),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
+                     "traffic": traffic,
+                     "kernel": "k_persist (persistent Rodas3, FP64 sparse; the algorithmic work "
+                               "per attempt from the mechanism tables)",
+                     "kernel_ms": round(k_ms, 3)},
+        "solver_stats": {
+            "learned": {"mean_attempts": float(nstp.mean()), "mean_rejects": float(L["st"][4].mean()),
+                        "failed_cells": int((L["ierr"] < 0).sum()),
+                        "rates_ms": round(float(np.mean([t[0] for t in L["tim"]])), 3)},
+            "plain": {"mean_attempts": float(P["st"][2].mean()), "mean_rejects": float(P["st"][4].mean()),
+                      "failed_cells": int((P["ierr"] < 0).sum()),
+                      "integrate_ms": round(float(np.mean([t[1] for t in P["tim"]])), 3)}},
+        "accuracy": accuracy,
+        "autotune_ms": [[round(float(v), 2) for v in row] for row in tune_ms],
+        "cpu_baseline": base,
+    }
+    ctx.close()
+    return out
+
+
+def mixed_bench(args):
+    """144x91x72 grid, fullchem/Hg/carbon 70/20/10 at random; one persistent
+    launch per step over all cells; 24 steps replayed from one HIP graph,
+    each step warm-started from the previous step's RSTATUS(3)."""
+    import numpy as np
+    import torch
+    from gckpp_amd import hip, mechanism, synth
+    from oracle.ref import icntrl_for
+    mechs = [mechanism.load(k) for k in ("fullchem", "Hg", "carbon")]
+    t0 = time.time()
+    parts, assign = synth.mixed_grid(mechs, 144, 91, 72, seed=3)
+    ntot = sum(p.ncell for p in parts)
+    log("mixed grid: %s cells (%.1f s)" % ([p.ncell for p in parts], time.time() - t0))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    ctxs = [hip.Context(m, p.ncell) for m, p in zip(mechs, parts)]
+    B = hip.Batch(*ctxs)
+    state = []
+    for m, p in zip(mechs, parts):
+        ic, rc = icntrl_for(m.name)
+        state.append(dict(met=dev(p.met), photol=dev(p.photol), het=dev(p.het), conc0=dev(p.conc),
+                          conc=dev(p.conc), hstart=torch.zeros(p.ncell, dtype=torch.float64, device="cuda"),
+                          istatus=torch.zeros((20, p.ncell), dtype=torch.int32, device="cuda"),
+                          ierr=torch.zeros(p.ncell, dtype=torch.int32, device="cuda"), ic=ic, rc=rc))
+
+    def parts_of():
+        return [B.part(met=s["met"], photol=s["photol"], het=s["het"], conc=s["conc"], hstart=s["hstart"],
+                       icntrl=s["ic"], rcntrl=s["rc"], istatus=s["istatus"], ierr=s["ierr"]) for s in state]
+
+    def reset():
+        for s in state:
+            s["conc"].copy_(s["conc0"])
+            s["hstart"].zero_()
+
+    # autotune: one cold step per configuration
+    tune = {}
+    for v in range(hip.PERSIST_VARIANTS):
+        for g in (1, 2, 4):
+            try:
+                B.set_config(v, g)
+            except hip.GckppError:
+                continue
+            reset()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            B.step(parts_of(), 1200.0)
+            torch.cuda.synchronize()
+            tune[(v, g)] = time.perf_counter() - t
+            log("mixed config v%d wpb%d: %.1f ms" % (v, g, tune[(v, g)] * 1e3))
+    best = min(tune, key=tune.get)
+    B.set_config(*best)
+    reset()
+    B.capture(parts_of(), 1200.0)
+    nsteps = 24
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(nsteps):
+        B.replay()
+        if k % 6 == 5:
+            torch.cuda.synchronize()
+            log("replayed %d steps (%.1f s)" % (k + 1, time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fails = sum(int((s["ierr"] < 0).sum().item()) for s in state)
+    out = {
+        "metric": "mixed-mechanism cells/sec per 1200 s chem step, 24 graph-replayed steps",
+        "value": round(ntot * nsteps / wall, 2), "unit": "cells/s", "n_gpus": 1, "steps": nsteps,
+        "warmup": 0, "ms_per_step": round(wall / nsteps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "144x91x72 grid (seed 3), fullchem/Hg/carbon %s cells assigned at "
+                               "random 70/20/10, one persistent launch per step, 24 steps from one "
+                               "HIP graph, warm start from RSTATUS(3)" % [p.ncell for p in parts],
+                   "persistent_config": {"variant": best[0], "waves_per_group": best[1]},
+                   "autotune_first_step_ms": {"v%d_wpb%d" % k: round(v * 1e3, 1) for k, v in tune.items()}},
+        "failed_cells_last_step": fails,
+        "mean_attempts_last_step": [float(s["istatus"][2].float().mean().item()) for s in state],
+    }
+    B.close()
+    for c in ctxs:
+        c.close()
+    return out
+
+
+def c90_bench(args):
+    """6x90x90x72 = 3,499,200 fullchem cells, one chemistry step, solved in
+    six face-sized chunks (583,200 cells) through one context whose
+    workspace fits HBM; learned h0 + persistent solver."""
+    import numpy as np
+    import torch
+    from gckpp_amd import hip, learned, mechanism, synth
+    from oracle.ref import fullchem_icntrl
+    mech = mechanism.load("fullchem")
+    ic, rc = fullchem_icntrl()
+    nface = 90 * 90 * 72
+    ctx = hip.Context(mech, nface)
+    ctx.set_solver(hip.SOLVER_PERSISTENT)
+    mpath = learned.default_model_path("fullchem")
+    model = learned.StepModel(mech).load(mpath) if mpath.exists() else None
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    faces = []
+    for f in range(6):
+        w = synth.grid_cells(mech, 90, 90, 72, seed=200 + f, hour_utc=(12.0 + 4.0 * f) % 24.0)
+        faces.append(dict(met=dev(w.met), photol=dev(w.photol), het=dev(w.het), conc0=dev(w.conc)))
+        log("face %d generated" % f)
+    h0 = torch.empty(nface, dtype=torch.float64, device="cuda")
+    cls = torch.empty(nface, dtype=torch.int32, device="cuda")
+    cold = torch.zeros(nface, dtype=torch.float64, device="cuda")
+    conc = torch.empty_like(faces[0]["conc0"])
+    fails, att = 0, []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in faces:
+        conc.copy_(f["conc0"])
+        hs = cold
+        if model is not None:
+            model.predict(f["met"], f["photol"], conc, safety=args.safety, h0=h0, cls=cls)
+            ctx.set_cell_order(hip.order_by_class(cls))
+            hs = h0
+        hnew, ist, ierr, ok = ctx.fullchem_step(f["met"], f["photol"], f["het"], conc, hs, 1200.0, ic, rc)
+        fails += int((ierr < 0).sum().item())
+        att.append(float(ist[2].float().mean().item()))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n = 6 * nface
+    return {"metric": METRIC + " (C90-sized)", "value": round(n / wall, 2), "unit": "cells/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "6x90x90x72 = 3,499,200 fullchem cells (six 90x90x72 synthetic "
+                                   "faces, seeds 200-205), one step, six 583,200-cell chunks, learned "
+                                   "h0 + persistent", "learned_model": model is not None},
+            "failed_cells": fails, "mean_attempts_per_face": att}
 
 
 def main():
@@ -82,91 +415,13 @@ def main():
     if args.copy_calibration:
         return copy_calibration()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the chemistry step has no "
-              "cross-GPU communication (cells shard independently), so run --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the chemistry step has no cross-GPU communication "
+              "(cells shard independently), so run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
-    from gckpp_amd import hip, mechanism, synth
-    from oracle.ref import fullchem_icntrl
-
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
-    mech = mechanism.load("fullchem")
-    nx, ny, nz = (int(v) for v in args.grid.split("x"))
-    w = synth.grid_cells(mech, nx, ny, nz, seed=1)
-    n = w.ncell
-    ic, rc = fullchem_icntrl()
-    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    met, photol, het, conc0, hstart = dev(w.met), dev(w.photol), dev(w.het), dev(w.conc), dev(w.hstart)
-    conc = conc0.clone()
-    hnew = torch.empty(n, dtype=torch.float64, device="cuda")
-    ist = torch.empty((20, n), dtype=torch.int32, device="cuda")
-    ierr = torch.empty(n, dtype=torch.int32, device="cuda")
-    ctx = hip.Context(mech, n)
-
-    def step():
-        conc.copy_(conc0)
-        return ctx.fullchem_step(met, photol, het, conc, hstart, w.dt, ic, rc, hnew=hnew,
-                                 istatus=ist, ierr=ierr)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t_int = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        t_int.append(ctx.step_timings())
-        print("step done: %.1f ms total, integrate %.1f ms" % (t_int[-1][3], t_int[-1][1]),
-              file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
-    cells_per_s = n / (wall / args.steps)
-
-    st = ist.cpu().numpy()
-    e = ierr.cpu().numpy()
-    nacc, nrej = st[3].astype(np.int64), st[4].astype(np.int64)
-    # attempts that are not counted as rejections (before the first accept)
-    nstp = st[2].astype(np.int64)
-    flop_int = FLOP_ACC * nacc.sum() + FLOP_REJ * (nstp - nacc).sum()
-    int_ms = float(np.mean([t[1] for t in t_int]))
-    achieved = flop_int / (int_ms * 1e-3) / 1e12
-    traffic = None
-    tfile = ROOT / "profiles" / "integrate_traffic.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-
-    out = {
-        "metric": METRIC,
-        "value": round(cells_per_s, 2),
-        "unit": "cells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "plain GPU Rosenbrock (Rodas3), fullchem, %dx%dx%d grid, "
-                               "%d cells, dt=1200 s, cold start, repo RTOL/ATOL" % (nx, ny, nz, n),
-                   "cells": n, "solver": "cell-per-lane Rodas3, table-driven"},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
-                     "traffic": traffic,
-                     "kernel": "k_integrate (Rodas3 integrate, FP64)",
-                     "kernel_ms": round(int_ms, 3)},
-        "solver_stats": {"mean_steps": float(nstp.mean()), "mean_rejects": float(nrej.mean()),
-                         "max_steps": int(nstp.max()), "failed_cells": int((e < 0).sum()),
-                         "rates_ms": round(float(np.mean([t[0] for t in t_int])), 3),
-                         "retry_ms": round(float(np.mean([t[2] for t in t_int])), 3)},
-        "cpu_baseline": cpu_baseline(mech, w, ic, rc, args.cpu_sample),
-    }
+    out = {"grid": grid_bench, "mixed": mixed_bench, "c90": c90_bench}[args.workload](args)
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

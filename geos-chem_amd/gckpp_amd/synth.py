@@ -236,6 +236,20 @@ def grid_cells(mech, nx=72, ny=46, nz=72, seed=1, dt=1200.0, nspec_fields=None, 
                                                "" if hour_utc == 12.0 else "_%gUTC" % hour_utc))
 
 
+def mixed_grid(mechs, nx=144, ny=91, nz=72, seed=3, split=(0.7, 0.2, 0.1), hour_utc=12.0):
+    """Cells of one nx*ny*nz grid assigned at random to the mechanisms mechs
+    (e.g. fullchem, Hg, carbon) with probabilities split; returns one
+    Workload per mechanism (its cells, in grid order) and the assignment."""
+    rng = np.random.default_rng(seed + 7919)
+    n = nx * ny * nz
+    assign = rng.choice(len(mechs), size=n, p=np.asarray(split) / np.sum(split))
+    parts = []
+    for k, m in enumerate(mechs):
+        idx = np.nonzero(assign == k)[0]
+        parts.append(grid_cells(m, nx, ny, nz, seed=seed, hour_utc=hour_utc).subset(idx))
+    return parts, assign
+
+
 def rich_het(rng, ncell):
     """Per-cell HetState records with cloud, ice and every aerosol type
     present, random pH, halide concentrations and flags: exercises every

@@ -5,7 +5,8 @@ It extracts only the mechanism *tables* -- species order, fixed species,
 reactant lists, net stoichiometry, rate-coefficient expressions and the sparse
 LU pattern -- which every solver for the mechanism must share so that it is a
 drop-in for the reference call.  No solver code is taken over: the kernels are
-generated from these tables by ``emit_hip.py`` and the oracle reads the JSON.
+generated from these tables by ``emit_tables.py`` and ``emit_rates.py``, and the
+oracle reads the JSON.
 
 Sources read (paths relative to the reference checkout, KPP/<mech>/):
   gckpp_Parameters.F90   NSPEC/NVAR/NFIX/NREACT/LU_NONZERO, ind_<name>

@@ -1,0 +1,83 @@
+! Fortran ISO_C_BINDING interface to libgckpp_hip.so (include/gckpp_hip.h):
+! the module a GEOS-Chem host binds to call the GPU chemistry solve in place
+! of Update_RCONST / Integrate (KPP/fullchem/gckpp_Rates.F90:408,
+! KPP/fullchem/gckpp_Integrator.F90:80).  Array arguments are the ABI's
+! [field][ncell] layout, i.e. Fortran A(ncell, NFIELD).
+MODULE gckpp_hip_mod
+  USE ISO_C_BINDING
+  IMPLICIT NONE
+  PRIVATE
+  INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_FULLCHEM = 0, GCKPP_HG = 1, GCKPP_CARBON = 2
+  INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_SOLVER_PLAIN = 0, GCKPP_SOLVER_PERSISTENT = 1
+  PUBLIC :: gckpp_init, gckpp_finalize, gckpp_set_tolerances, gckpp_set_species_consts
+  PUBLIC :: gckpp_set_solver, gckpp_fullchem_step_dev, gckpp_integrate, gckpp_last_error
+  PUBLIC :: gckpp_mech_sizes
+
+  INTERFACE
+    ! out = NSPEC, NVAR, NFIX, NREACT, LU_NONZERO, NMET, NHET, NPHOTOL (no GPU needed)
+    INTEGER(C_INT) FUNCTION gckpp_mech_sizes(mech_id, out) BIND(C, name="gckpp_mech_sizes")
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT),     VALUE       :: mech_id
+      INTEGER(C_INT32_T), INTENT(OUT) :: out(8)
+    END FUNCTION
+    ! int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out)
+    INTEGER(C_INT) FUNCTION gckpp_init(mech_id, max_cells, ctx) BIND(C, name="gckpp_init")
+      IMPORT :: C_INT, C_INT64_T, C_PTR
+      INTEGER(C_INT),     VALUE :: mech_id
+      INTEGER(C_INT64_T), VALUE :: max_cells
+      TYPE(C_PTR)               :: ctx            ! out
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION gckpp_finalize(ctx) BIND(C, name="gckpp_finalize")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    ! ATOL(1:NVAR), RTOL(1:NVAR)
+    INTEGER(C_INT) FUNCTION gckpp_set_tolerances(ctx, atol, rtol) BIND(C, name="gckpp_set_tolerances")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE         :: ctx
+      REAL(C_DOUBLE), INTENT(IN) :: atol(*), rtol(*)
+    END FUNCTION
+    ! MW, HENRY_K0, HENRY_CR (1:NSPEC)
+    INTEGER(C_INT) FUNCTION gckpp_set_species_consts(ctx, mw, hk0, hcr) &
+         BIND(C, name="gckpp_set_species_consts")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE         :: ctx
+      REAL(C_DOUBLE), INTENT(IN) :: mw(*), hk0(*), hcr(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION gckpp_set_solver(ctx, solver) BIND(C, name="gckpp_set_solver")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR),    VALUE :: ctx
+      INTEGER(C_INT), VALUE :: solver
+    END FUNCTION
+    ! One chemistry step of ncell cells (device pointers, HIP stream)
+    INTEGER(C_INT) FUNCTION gckpp_fullchem_step_dev(ctx, ncell, dt, icntrl, rcntrl, met, photol, &
+         het, het_stride, hstart, conc, hnew, istatus, ierr, stream)                            &
+         BIND(C, name="gckpp_fullchem_step_dev")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR),        VALUE      :: ctx
+      INTEGER(C_INT64_T), VALUE      :: ncell, het_stride
+      REAL(C_DOUBLE),     VALUE      :: dt
+      INTEGER(C_INT32_T), INTENT(IN) :: icntrl(20)
+      REAL(C_DOUBLE),     INTENT(IN) :: rcntrl(20)
+      TYPE(C_PTR),        VALUE      :: met, photol, het, hstart, conc, hnew, istatus, ierr, stream
+    END FUNCTION
+    ! Host-pointer Integrate (synchronous): C(ncell,NSPEC), RCONST(ncell,NREACT), ...
+    INTEGER(C_INT) FUNCTION gckpp_integrate(ctx, ncell, tin, tout, icntrl, rcntrl, hstart, conc, &
+         rconst, istatus, rstatus, ierr) BIND(C, name="gckpp_integrate")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR),        VALUE         :: ctx
+      INTEGER(C_INT64_T), VALUE         :: ncell
+      REAL(C_DOUBLE),     VALUE         :: tin, tout
+      INTEGER(C_INT32_T), INTENT(IN)    :: icntrl(20)
+      REAL(C_DOUBLE),     INTENT(IN)    :: rcntrl(20)
+      REAL(C_DOUBLE),     INTENT(IN)    :: hstart(*)
+      REAL(C_DOUBLE),     INTENT(INOUT) :: conc(*)
+      REAL(C_DOUBLE),     INTENT(IN)    :: rconst(*)
+      INTEGER(C_INT32_T), INTENT(OUT)   :: istatus(*), ierr(*)
+      REAL(C_DOUBLE),     INTENT(OUT)   :: rstatus(*)
+    END FUNCTION
+    TYPE(C_PTR) FUNCTION gckpp_last_error() BIND(C, name="gckpp_last_error")
+      IMPORT :: C_PTR
+    END FUNCTION
+  END INTERFACE
+END MODULE gckpp_hip_mod

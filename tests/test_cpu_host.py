@@ -121,3 +121,28 @@ def test_grid_workload_shape(fullchem):
     assert (np.diff(P[:, 0, 0]) < 0).all()         # pressure falls with level
     lit = w.met[fields.MET["SUNCOS"][0]] > 0
     assert 0.2 < lit.mean() < 0.8
+
+
+def test_fortran_binding_compiles_and_links(tmp_path):
+    """include/gckpp_hip_mod.F90 (the ISO_C_BINDING module of INTEGRATION)
+    compiles with flang, links against the library and calls
+    gckpp_mech_sizes, which needs no GPU."""
+    import shutil
+    import subprocess
+    from gckpp_amd import hip
+    flang = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+    if not Path(flang).exists() or not hip.lib_path().exists():
+        pytest.skip("flang or the HIP library not available")
+    root = Path(__file__).resolve().parent.parent
+    prog = tmp_path / "t.F90"
+    prog.write_text(
+        "program t\n  use gckpp_hip_mod\n  use iso_c_binding\n  integer(c_int32_t) :: s(8)\n"
+        "  integer(c_int) :: rc\n  rc = gckpp_mech_sizes(GCKPP_FULLCHEM, s)\n"
+        "  print '(i0,8(1x,i0))', rc, s\nend program t\n")
+    libdir = hip.lib_path().parent
+    exe = tmp_path / "t"
+    subprocess.run([flang, "-o", str(exe), str(root / "include" / "gckpp_hip_mod.F90"), str(prog),
+                    "-J", str(tmp_path), "-L" + str(libdir), "-lgckpp_hip",
+                    "-Wl,-rpath," + str(libdir)], check=True, cwd=tmp_path)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert [int(v) for v in out[:6]] == [0, 356, 353, 3, 1058, 5683]
