@@ -164,11 +164,10 @@ def grid_bench(args):
     else:
         log("no trained model at %s: the learned leg runs with cold Hstart" % mpath)
 
-    # autotune the persistent solver on every 7th cell
-    sub = torch.arange(0, n, 7, device="cuda")
+    # autotune the persistent solver on the workload itself (a sample smaller
+    # than the resident lanes would time a different, latency-only regime)
     rct = ctx.rates(met, photol, het, conc0)
-    tune_ms = ctx.autotune(conc0[:, sub].contiguous(), rct[:, sub].contiguous(), cold[sub].contiguous(),
-                           0.0, w.dt, ic, rc)
+    tune_ms = ctx.autotune(conc0, rct, cold, 0.0, w.dt, ic, rc)
     del rct
     pcfg = ctx.persistent_config()
     log("autotuned persistent config %s" % pcfg)

@@ -648,7 +648,7 @@ extern "C" int gckpp_set_step_log(gckpp_ctx *c, float *hlog, int nattempt) {
   return 0;
 }
 
-extern "C" int gckpp_get_persistent_config(gckpp_ctx *c, int32_t out[5]) {
+extern "C" int gckpp_get_persistent_config(gckpp_ctx *c, int32_t out[6]) {
   if (!c || !out) return fail("gckpp_get_persistent_config: null argument");
   int U = 0, W = 0;
   persist_variant_info(c->pvariant, &U, &W);
@@ -657,8 +657,11 @@ extern "C" int gckpp_get_persistent_config(gckpp_ctx *c, int32_t out[5]) {
   out[2] = (int32_t)c->pwaves;
   out[3] = U;
   out[4] = W;
+  out[5] = persist_variant_seg(c->pvariant);
   return 0;
 }
+
+extern "C" int gckpp_persistent_variants(void) { return PERSIST_VARIANTS; }
 
 // Times the persistent solver on the caller's cells for every variant and
 // workgroup size, keeps the fastest, and leaves conc untouched (each trial

@@ -384,12 +384,18 @@ __global__ __launch_bounds__(256) void k_class_hist(int64_t n, const int32_t *cl
 }
 
 // In place: hist[b*4+k] <- start of block b's class-k cells in the output.
+// Thread t owns the contiguous blocks [t*per, (t+1)*per): key by key, the
+// start of its first block is the totals of all earlier keys plus the
+// totals of key k over the blocks of threads before it.
 __global__ __launch_bounds__(256) void k_class_scan(int64_t nb, int32_t *hist) {
   __shared__ int32_t tot[4][256];
   __shared__ int32_t base[4];
+  const int64_t per = (nb + 255) / 256;
+  const int64_t b0 = (int64_t)threadIdx.x * per;
+  const int64_t b1 = b0 + per < nb ? b0 + per : nb;
   for (int k = 0; k < 4; ++k) {
     int32_t s = 0;
-    for (int64_t b = threadIdx.x; b < nb; b += 256) s += hist[b * 4 + k];
+    for (int64_t b = b0; b < b1; ++b) s += hist[b * 4 + k];
     tot[k][threadIdx.x] = s;
   }
   __syncthreads();
@@ -401,11 +407,10 @@ __global__ __launch_bounds__(256) void k_class_scan(int64_t nb, int32_t *hist) {
     }
   }
   __syncthreads();
-  // thread t owns blocks t, t+256, ...: its start is base + totals of earlier threads
   for (int k = 0; k < 4; ++k) {
     int32_t run = base[k];
     for (int t = 0; t < (int)threadIdx.x; ++t) run += tot[k][t];
-    for (int64_t b = threadIdx.x; b < nb; b += 256) {
+    for (int64_t b = b0; b < b1; ++b) {
       int32_t c = hist[b * 4 + k];
       hist[b * 4 + k] = run;
       run += c;

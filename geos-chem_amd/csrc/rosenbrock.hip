@@ -36,6 +36,29 @@ struct CCol {
   __device__ double operator[](int k) const { return p[(int64_t)k * stride]; }
 };
 
+// Per-lane view of one wave's slot block through a buffer resource: element
+// k of the field starting at byte offset base is at base + k*512 (64 lanes x
+// 8 bytes), read and written with buffer instructions whose wave-uniform part
+// of the address is an SGPR offset, so a load needs no address VGPRs.
+typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
+struct BRef {
+  __amdgpu_buffer_rsrc_t r;
+  int voff, soff;
+  __device__ operator double() const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
+  __device__ const BRef &operator=(double v) const {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+    return *this;
+  }
+  __device__ const BRef &operator+=(double v) const { return *this = (double)*this + v; }
+};
+struct BCol {
+  __amdgpu_buffer_rsrc_t r;
+  int voff, base;
+  __device__ BRef operator[](int k) const { return BRef{r, voff, base + k * 512}; }
+};
+
 // ---------------------------------------------------------------------------
 // Mechanism-generic device pieces.  M is one of the mech_<name> namespaces,
 // wrapped in a traits struct so it can be a template argument.
@@ -69,15 +92,30 @@ struct CCol {
     static __device__ int cr_l(int i) { return mech_##NS::cr_l[i]; }           \
     static __device__ int cr_u(int i) { return mech_##NS::cr_u[i]; }           \
     static __device__ int cr_div(int i) { return mech_##NS::cr_div[i]; }       \
+    static __device__ int cu_row(int i) { return mech_##NS::cu_row[i]; }       \
+    static __device__ int cu_sp(int i) { return mech_##NS::cu_sp[i]; }         \
+    static __device__ int cu_q(int i) { return mech_##NS::cu_q[i]; }           \
+    static __device__ int cu_l(int i) { return mech_##NS::cu_l[i]; }           \
+    static __device__ int cu_u(int i) { return mech_##NS::cu_u[i]; }           \
+    static __device__ int sf_sp(int i) { return mech_##NS::sf_sp[i]; }         \
+    static __device__ int sf_l(int i) { return mech_##NS::sf_l[i]; }           \
+    static __device__ int sf_x(int i) { return mech_##NS::sf_x[i]; }           \
+    static __device__ int sf_batch(int i) { return mech_##NS::sf_batch[i]; }   \
+    static __device__ int sb_sp(int i) { return mech_##NS::sb_sp[i]; }         \
+    static __device__ int sb_l(int i) { return mech_##NS::sb_l[i]; }           \
+    static __device__ int sb_x(int i) { return mech_##NS::sb_x[i]; }           \
+    static __device__ int sb_batch(int i) { return mech_##NS::sb_batch[i]; }   \
+    static constexpr int NSF_BATCH = mech_##NS::NSF_BATCH;                     \
+    static constexpr int NSB_BATCH = mech_##NS::NSB_BATCH;                     \
   };
 GCKPP_MECH_TRAITS(fullchem)
 GCKPP_MECH_TRAITS(Hg)
 GCKPP_MECH_TRAITS(carbon)
 
 // Concentration of species s (variable from y, fixed from the conc array).
-template <class M>
-__device__ inline double spc(const CCol &y, const CCol &fix, int s) {
-  return s < M::NVAR ? y[s] : fix[s];
+template <class M, class CY, class CF>
+__device__ inline double spc(const CY &y, const CF &fix, int s) {
+  return s < M::NVAR ? (double)y[s] : (double)fix[s];
 }
 
 // acc + sum_{t0<=t<t1} w(t)*v(t), added in increasing t.  With U > 1 the
@@ -87,6 +125,15 @@ __device__ inline double spc(const CCol &y, const CCol &fix, int s) {
 template <int U, class W, class V>
 __device__ __forceinline__ double gather_fma(int t0, int t1, double acc, W w, V v) {
   int t = t0;
+  if constexpr (U >= 16) {
+    for (; t + 16 <= t1; t += 16) {
+      double a[16], b[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) { a[u] = w(t + u); b[u] = v(t + u); }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) acc = fma(a[u], b[u], acc);
+    }
+  }
   if constexpr (U >= 8) {
     for (; t + 8 <= t1; t += 8) {
       double a[8], b[8];
@@ -116,9 +163,101 @@ __device__ __forceinline__ double gather_fma(int t0, int t1, double acc, W w, V 
   return acc;
 }
 
+// Segment sums over a flat term list: segment s (s0 <= s < s1) covers terms
+// [sp(s), sp(s+1)); its value, sum_t w(t)*v(t) accumulated in term order from
+// 0, goes to fin(s, value) when the segment ends (empty segments give 0).
+// Terms go in chunks of C whose loads are all issued before the chunk's FMAs
+// and stores, so a wave waits once per chunk instead of once per segment.
+// The segments of one call must not read each other's results.
+template <int C, class SP, class W, class V, class FIN>
+__device__ __forceinline__ void seg_gather(int s0, int s1, SP sp, W w, V v, FIN fin) {
+  if (s0 >= s1) return;
+  int s = s0, send = 0;
+  auto next = [&]() {  // skip (and finish) empty segments
+#pragma unroll 1
+    while (s < s1) {
+      send = sp(s + 1);
+      if (send != sp(s)) break;
+      fin(s, 0.0);
+      ++s;
+    }
+  };
+  next();
+  int t = sp(s0);
+  const int tend = sp(s1);
+  double acc = 0.0;
+#pragma unroll 1
+  while (t < tend) {
+    const int n = tend - t < C ? tend - t : C;
+    double a[C], b[C];
+#pragma unroll
+    for (int u = 0; u < C; ++u)
+      if (u < n) {
+        a[u] = w(t + u);
+        b[u] = v(t + u);
+      }
+#pragma unroll
+    for (int u = 0; u < C; ++u)
+      if (u < n) {
+        acc = fma(a[u], b[u], acc);
+        if (t + u + 1 == send) {
+          fin(s, acc);
+          acc = 0.0;
+          ++s;
+          next();
+        }
+      }
+    t += n;
+  }
+}
+
+// st(i, f(i)) for 0 <= i < n, B values computed (loads only) before their B
+// stores; the f(i) must not read what st writes.
+template <int B, class F, class ST>
+__device__ __forceinline__ void vbatch(int n, F f, ST st) {
+  int i = 0;
+  if constexpr (B > 1) {
+#pragma unroll 1
+    for (; i + B <= n; i += B) {
+      double v[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) v[u] = f(i + u);
+#pragma unroll
+      for (int u = 0; u < B; ++u) st(i + u, v[u]);
+    }
+  }
+#pragma unroll 1
+  for (; i < n; ++i) st(i, f(i));
+}
+
+
 // A(r) = RCT(r)*prod(reactants) into a[], then Vdot(i) = sum c*A(r).
-template <class M, int U = 1>
-__device__ void dev_fun(const CCol &y, const CCol &fix, const CCol &rct, Col a, Col vdot) {
+template <class M, int U = 1, bool SEG = false, class CY, class CF, class CR, class CA, class CV>
+__device__ void dev_fun(const CY &y, const CF &fix, const CR &rct, CA a, CV vdot) {
+  if constexpr (SEG) {
+    // operands of 4 reactions loaded, then their products (same order)
+    vbatch<4>(
+        M::NREACT,
+        [&](int r) -> double {
+          double o[M::RX_MAX];
+#pragma unroll
+          for (int q = 0; q < M::RX_MAX; ++q) {
+            int s = M::rx_s(r * M::RX_MAX + q);
+            o[q] = s < 0 ? 1.0 : spc<M>(y, fix, s);
+          }
+          double v = rct[r];
+#pragma unroll
+          for (int q = 0; q < M::RX_MAX; ++q)
+            if (M::rx_s(r * M::RX_MAX + q) >= 0) v *= o[q];
+          return v;
+        },
+        [&](int r, double v) { a[r] = v; });
+    seg_gather<(U < 8 ? U : 8)>(0, M::NVAR, [&](int i) { return M::fv_ptr(i); },
+                  [&](int t) -> double { return M::fv_c(t); },
+                  [&](int t) -> double { return a[M::fv_r(t)]; },
+                  [&](int i, double acc) { vdot[i] = acc; });
+    return;
+  }
 #pragma unroll 2
   for (int r = 0; r < M::NREACT; ++r) {
     double v = rct[r];
@@ -133,16 +272,42 @@ __device__ void dev_fun(const CCol &y, const CCol &fix, const CCol &rct, Col a, 
 #pragma unroll 1
   for (int i = 0; i < M::NVAR; ++i)
     vdot[i] = gather_fma<U>(M::fv_ptr(i), M::fv_ptr(i + 1), 0.0,
-                            [&](int t) { return M::fv_c(t); },
-                            [&](int t) { return a[M::fv_r(t)]; });
+                            [&](int t) -> double { return M::fv_c(t); },
+                            [&](int t) -> double { return a[M::fv_r(t)]; });
 }
 
 // JVS in LU layout: partials into b[], then each LU entry gathers its terms;
 // out[k] = sign*JVS(k) (+ diag on the diagonal), so sign=-1, diag=1/(H*gamma)
 // builds ros_PrepareMatrix's Ghimj directly.
-template <class M, int U = 1>
-__device__ void dev_jac(const CCol &y, const CCol &fix, const CCol &rct, Col b, Col out,
-                        double sign, double diag) {
+template <class M, int U = 1, bool SEG = false, class CY, class CF, class CR, class CB, class CO>
+__device__ void dev_jac(const CY &y, const CF &fix, const CR &rct, CB b, CO out, double sign,
+                        double diag) {
+  if constexpr (SEG) {
+    vbatch<4>(
+        M::NPART,
+        [&](int p) -> double {
+          double o[M::RX_MAX - 1];
+#pragma unroll
+          for (int q = 0; q < M::RX_MAX - 1; ++q) {
+            int s = M::pt_s(p * (M::RX_MAX - 1) + q);
+            o[q] = s < 0 ? 1.0 : spc<M>(y, fix, s);
+          }
+          double v = M::pt_m(p) * (double)rct[M::pt_r(p)];
+#pragma unroll
+          for (int q = 0; q < M::RX_MAX - 1; ++q)
+            if (M::pt_s(p * (M::RX_MAX - 1) + q) >= 0) v *= o[q];
+          return v;
+        },
+        [&](int p, double v) { b[p] = v; });
+    seg_gather<(U < 8 ? U : 8)>(0, M::LU_NZ, [&](int k) { return M::jv_ptr(k); },
+                  [&](int t) -> double { return M::jv_c(t); },
+                  [&](int t) -> double { return b[M::jv_b(t)]; },
+                  [&](int k, double acc) { out[k] = sign * acc; });
+    if (diag != 0.0)
+      vbatch<8>(M::NVAR, [&](int i) -> double { return (double)out[M::lu_diag(i)] + diag; },
+                [&](int i, double v) { out[M::lu_diag(i)] = v; });
+    return;
+  }
 #pragma unroll 2
   for (int p = 0; p < M::NPART; ++p) {
     double v = M::pt_m(p) * rct[M::pt_r(p)];
@@ -157,8 +322,8 @@ __device__ void dev_jac(const CCol &y, const CCol &fix, const CCol &rct, Col b, 
 #pragma unroll 1
   for (int k = 0; k < M::LU_NZ; ++k) {
     double acc = gather_fma<U>(M::jv_ptr(k), M::jv_ptr(k + 1), 0.0,
-                               [&](int t) { return M::jv_c(t); },
-                               [&](int t) { return b[M::jv_b(t)]; });
+                               [&](int t) -> double { return M::jv_c(t); },
+                               [&](int t) -> double { return b[M::jv_b(t)]; });
     out[k] = sign * acc;
   }
   if (diag != 0.0)
@@ -193,43 +358,81 @@ __device__ int dev_decomp(Col lu) {
 // pivots j < min(c,k) in increasing j -- the same updates, in the same order,
 // as the row-wise elimination -- then is divided by U(c,c) when c < k.  Each
 // position is written once and its partial sum stays in a register.
-template <class M, int U = 1>
-__device__ int dev_decomp_crout(Col lu) {
+template <class M, int U = 1, bool SEG = false, class CL>
+__device__ int dev_decomp_crout(CL lu) {
   int ier = 0;
   int q = 0;
 #pragma unroll 1
   for (int k = 0; k < M::NVAR; ++k) {
     if (ier == 0 && fabs(lu[M::lu_diag(k)]) < DBL_MIN) ier = k + 1;
     const int q1 = M::lu_crow(k + 1);
+    // with SEG only the L part goes position by position (each position
+    // reads the row's earlier L entries); the U part, whose positions are
+    // independent, goes as flat segments (first term: lu[q] times
+    // -lu[LU_NZ] = 1, the slot's constant entry)
+    const int qe = SEG ? M::lu_diag(k) : q1;
 #pragma unroll 1
-    for (; q < q1; ++q) {
-      double acc = gather_fma<U>(M::cr_ptr(q), M::cr_ptr(q + 1), lu[q],
-                                 [&](int t) { return -lu[M::cr_l(t)]; },
-                                 [&](int t) { return lu[M::cr_u(t)]; });
+    for (; q < qe; ++q) {
+      double acc = gather_fma<U>(M::cr_ptr(q), M::cr_ptr(q + 1), (double)lu[q],
+                                 [&](int t) -> double { return -(double)lu[M::cr_l(t)]; },
+                                 [&](int t) -> double { return lu[M::cr_u(t)]; });
       const int dv = M::cr_div(q);
       if (dv >= 0) acc = acc / lu[dv];
       lu[q] = acc;
+    }
+    if constexpr (SEG) {
+      seg_gather<(U < 8 ? U : 8)>(M::cu_row(k), M::cu_row(k + 1), [&](int sg) { return M::cu_sp(sg); },
+                    [&](int t) -> double { return -(double)lu[M::cu_l(t)]; },
+                    [&](int t) -> double { return lu[M::cu_u(t)]; },
+                    [&](int sg, double acc) { lu[M::cu_q(sg)] = acc; });
+      q = q1;
     }
   }
   return ier;
 }
 
 // KppSolve in place: unit-lower forward then upper backward substitution.
-template <class M, int U = 1>
-__device__ void dev_solve(const CCol &lu, Col x) {
+template <class M, int U = 1, bool SEG = false, class CL, class CX>
+__device__ void dev_solve(const CL &lu, CX x) {
+  if constexpr (SEG) {
+    // forward: batches of rows that do not read each other; backward: the
+    // numerators of a batch, then their divisions by the pivots
+#pragma unroll 1
+    for (int bt = 0; bt < M::NSF_BATCH; ++bt)
+      seg_gather<(U < 8 ? U : 8)>(M::sf_batch(bt), M::sf_batch(bt + 1), [&](int sg) { return M::sf_sp(sg); },
+                    [&](int t) -> double { return -(double)lu[M::sf_l(t)]; },
+                    [&](int t) -> double { return x[M::sf_x(t)]; },
+                    [&](int sg, double acc) { x[sg] = acc; });
+#pragma unroll 1
+    for (int bt = 0; bt < M::NSB_BATCH; ++bt) {
+      const int s0 = M::sb_batch(bt), s1 = M::sb_batch(bt + 1);
+      seg_gather<(U < 8 ? U : 8)>(s0, s1, [&](int sg) { return M::sb_sp(sg); },
+                    [&](int t) -> double { return -(double)lu[M::sb_l(t)]; },
+                    [&](int t) -> double { return x[M::sb_x(t)]; },
+                    [&](int sg, double acc) { x[M::NVAR - 1 - sg] = acc; });
+      vbatch<8>(
+          s1 - s0,
+          [&](int j) -> double {
+            const int i = M::NVAR - 1 - (s0 + j);
+            return (double)x[i] / (double)lu[M::lu_diag(i)];
+          },
+          [&](int j, double v) { x[M::NVAR - 1 - (s0 + j)] = v; });
+    }
+    return;
+  }
 #pragma unroll 1
   for (int i = 0; i < M::NVAR; ++i) {
     int k0 = M::lu_crow(i), k1 = M::lu_diag(i);
     if (k0 == k1) continue;
-    x[i] = gather_fma<U>(k0, k1, x[i], [&](int k) { return -lu[k]; },
-                         [&](int k) { return x[M::lu_icol(k)]; });
+    x[i] = gather_fma<U>(k0, k1, (double)x[i], [&](int k) -> double { return -(double)lu[k]; },
+                         [&](int k) -> double { return x[M::lu_icol(k)]; });
   }
 #pragma unroll 1
   for (int i = M::NVAR - 1; i >= 0; --i) {
     int kd = M::lu_diag(i), k1 = M::lu_crow(i + 1);
-    double acc = gather_fma<U>(kd + 1, k1, x[i], [&](int k) { return -lu[k]; },
-                               [&](int k) { return x[M::lu_icol(k)]; });
-    x[i] = acc / lu[kd];
+    double acc = gather_fma<U>(kd + 1, k1, (double)x[i], [&](int k) -> double { return -(double)lu[k]; },
+                               [&](int k) -> double { return x[M::lu_icol(k)]; });
+    x[i] = acc / (double)lu[kd];
   }
 }
 
@@ -420,8 +623,8 @@ struct SlotLayout {
   static constexpr int K = F0 + M::NVAR;            // 4*NVAR stage vectors
   static constexpr int YN = K + 4 * M::NVAR;        // NVAR Ynew
   static constexpr int S = YN + M::NVAR;            // SCR reaction rates / partials
-  static constexpr int LU = S + SCR;                // LU_NZ
-  static constexpr int FIELDS = LU + M::LU_NZ;
+  static constexpr int LU = S + SCR;                // LU_NZ, then the constant -1
+  static constexpr int FIELDS = LU + M::LU_NZ + 1;
 };
 
 // Register state of one lane's cell.
@@ -432,16 +635,25 @@ struct LaneState {
   bool rejLast, rejMore, newStep;
 };
 
+// One lane's slot: the wave's slot block as a buffer resource (SGPRs) and
+// the lane's byte offset in it; field f of the lane is at f*512 + voff.
+struct Slot {
+  __amdgpu_buffer_rsrc_t r;
+  int voff;
+  __device__ BCol col(int field) const { return BCol{r, voff, field * 512}; }
+};
+
 template <class M>
-__device__ __forceinline__ void lane_begin(const IntegrateArgs &a, double *blk, int64_t cell,
+__device__ __forceinline__ void lane_begin(const IntegrateArgs &a, const Slot &sl, int64_t cell,
                                            bool cold, LaneState &S) {
   using L = SlotLayout<M>;
   const RosParams &P = a.prm;
+  const BCol Y = sl.col(L::Y), R = sl.col(L::R);
+  sl.col(L::LU)[M::LU_NZ] = -1.0;  // the segment tables' weight-one entry
   #pragma unroll 4
-  for (int s = 0; s < M::NSPEC; ++s) blk[(int64_t)(L::Y + s) * 64] = a.conc[(int64_t)s * a.cstride + cell];
+  for (int s = 0; s < M::NSPEC; ++s) Y[s] = a.conc[(int64_t)s * a.cstride + cell];
   #pragma unroll 4
-  for (int r = 0; r < M::NREACT; ++r)
-    blk[(int64_t)(L::R + r) * 64] = a.rconst[(int64_t)r * a.rstride + cell];
+  for (int r = 0; r < M::NREACT; ++r) R[r] = a.rconst[(int64_t)r * a.rstride + cell];
   double hs = (a.hstart && !cold) ? a.hstart[cell] : 0.0;
   double Hstart = P.Hstart_default;
   if (hs > 0.0) Hstart = min_f(fabs(hs), fabs(P.Tend - P.Tstart));
@@ -459,17 +671,17 @@ __device__ __forceinline__ void lane_begin(const IntegrateArgs &a, double *blk, 
 
 // One Rodas3 attempt (or the end-of-integration checks); true when the cell
 // is done (S.ierr then holds IERR).
-template <class M, int U>
-__device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, double *blk, int64_t cell,
+template <class M, int U, bool SEG>
+__device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, const Slot &sl, int64_t cell,
                                              LaneState &S) {
   using L = SlotLayout<M>;
   const int N = M::NVAR;
-  auto F = [&](int off) { return Col{blk + (int64_t)off * 64, 64}; };
-  auto Fc = [&](int off) { return CCol{blk + (int64_t)off * 64, 64}; };
-  Col Y = F(L::Y), Fcn0 = F(L::F0), Ynew = F(L::YN), Ascr = F(L::S), LU = F(L::LU);
-  CCol Yc = Fc(L::Y), rct = Fc(L::R), Ynewc = Fc(L::YN);
-  auto Ks = [&](int s) { return F(L::K + s * N); };
-  auto Ksc = [&](int s) { return Fc(L::K + s * N); };
+  constexpr int VB = SEG ? 4 : 1;
+  const BCol Y = sl.col(L::Y), Fcn0 = sl.col(L::F0), Ynew = sl.col(L::YN), Ascr = sl.col(L::S),
+             LU = sl.col(L::LU), rct = sl.col(L::R);
+  const BCol &Yc = Y, &Ynewc = Ynew;
+  auto Ks = [&](int s) { return sl.col(L::K + s * N); };
+  auto Ksc = [&](int s) { return sl.col(L::K + s * N); };
   const RosParams &P = a.prm;
   const double Tend = P.Tend, dir = P.direction;
 
@@ -479,15 +691,15 @@ __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, double *blk
     if (S.st[2] > P.max_steps) { S.ierr = -6; return true; }
     if ((S.T + 0.1 * S.H) == S.T || S.H <= P.roundoff) { S.ierr = -7; return true; }
     S.H = min_f(S.H, fabs(Tend - S.T));
-    dev_fun<M, U>(Yc, Yc, rct, Ascr, Fcn0);
+    dev_fun<M, U, SEG>(Yc, Yc, rct, Ascr, Fcn0);
     S.st[0] += P.autonomous ? 1 : 2;
     S.st[1] += 1;
   }
   int nconsec = 0;
   bool singular = true;
   while (singular) {
-    dev_jac<M, U>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * S.H * kRosGamma1));
-    int ising = dev_decomp_crout<M, U>(LU);
+    dev_jac<M, U, SEG>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * S.H * kRosGamma1));
+    int ising = dev_decomp_crout<M, U, SEG>(LU);
     S.st[5] += 1;
     if (ising == 0) {
       singular = false;
@@ -501,47 +713,72 @@ __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, double *blk
   if (singular) { S.ierr = -8; return true; }
   const double H = S.H;
   for (int s = 0; s < 4; ++s) {
-    Col Kc = Ks(s);
+    const BCol Kc = Ks(s);
     if (s == 0 || !kRosNewF[s]) {
-      #pragma unroll 4
-      for (int i = 0; i < N; ++i) Kc[i] = Fcn0[i];
+      vbatch<VB>(N, [&](int i) -> double { return Fcn0[i]; }, [&](int i, double v) { Kc[i] = v; });
     } else {
-      #pragma unroll 4
-      for (int i = 0; i < N; ++i) {
-        double v = Yc[i];
-        for (int j = 0; j < s; ++j) {
-          double aj = kRosA[s * (s - 1) / 2 + j];
-          if (aj != 0.0) v = fma(aj, Ksc(j)[i], v);
-        }
-        Ynew[i] = v;
-      }
-      dev_fun<M, U>(Ynewc, Yc, rct, Ascr, Kc);
+      vbatch<VB>(
+          N,
+          [&](int i) -> double {
+            double v = Yc[i];
+            for (int j = 0; j < s; ++j) {
+              double aj = kRosA[s * (s - 1) / 2 + j];
+              if (aj != 0.0) v = fma(aj, (double)Ksc(j)[i], v);
+            }
+            return v;
+          },
+          [&](int i, double v) { Ynew[i] = v; });
+      dev_fun<M, U, SEG>(Ynewc, Yc, rct, Ascr, Kc);
       S.st[0] += 1;
     }
     for (int j = 0; j < s; ++j) {
       double hc = kRosC[s * (s - 1) / 2 + j] / (dir * H);
-      if (hc != 0.0)
-        #pragma unroll 4
-        for (int i = 0; i < N; ++i) Kc[i] = fma(hc, Ksc(j)[i], Kc[i]);
+      if (hc != 0.0) {
+        const BCol Kj = Ksc(j);
+        vbatch<VB>(N, [&](int i) -> double { return fma(hc, (double)Kj[i], (double)Kc[i]); },
+                   [&](int i, double v) { Kc[i] = v; });
+      }
     }
-    dev_solve<M, U>(CCol{LU.p, 64}, Kc);
+    dev_solve<M, U, SEG>(LU, Kc);
     S.st[6] += 1;
   }
   double err = 0.0;
-  #pragma unroll 4
-  for (int i = 0; i < N; ++i) {
+  // Ynew and the scaled error terms of VB species, then the stores and the
+  // error sum in species order
+  auto err_term = [&](int i, double &v, double &qq) {
     double y0 = Yc[i];
-    double v = y0;
+    v = y0;
     for (int j = 0; j < 4; ++j)
-      if (kRosM[j] != 0.0) v = fma(kRosM[j], Ksc(j)[i], v);
-    Ynew[i] = v;
+      if (kRosM[j] != 0.0) v = fma(kRosM[j], (double)Ksc(j)[i], v);
     double ye = 0.0;
     for (int j = 0; j < 4; ++j)
-      if (kRosE[j] != 0.0) ye = fma(kRosE[j], Ksc(j)[i], ye);
+      if (kRosE[j] != 0.0) ye = fma(kRosE[j], (double)Ksc(j)[i], ye);
     double ymax = max_f(fabs(y0), fabs(v));
     double sc = P.vectol ? a.atol[i] + a.rtol[i] * ymax : a.atol[0] + a.rtol[0] * ymax;
-    double q = ye / sc;
-    err = fma(q, q, err);
+    qq = ye / sc;
+  };
+  {
+    int i = 0;
+    if constexpr (VB > 1) {
+#pragma unroll 1
+      for (; i + VB <= N; i += VB) {
+        double v[VB], qq[VB];
+#pragma unroll
+        for (int u = 0; u < VB; ++u) err_term(i + u, v[u], qq[u]);
+#pragma unroll
+        for (int u = 0; u < VB; ++u) {
+          Ynew[i + u] = v[u];
+          err = fma(qq[u], qq[u], err);
+        }
+      }
+    }
+#pragma unroll 1
+    for (; i < N; ++i) {
+      double v, qq;
+      err_term(i, v, qq);
+      Ynew[i] = v;
+      err = fma(qq, qq, err);
+    }
   }
   err = max_f(sqrt(err / N), 1.0e-10);
   double fac = min_f(P.FacMax, max_f(P.FacMin, P.FacSafe / pow(err, 1.0 / kRosELO)));
@@ -554,11 +791,10 @@ __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, double *blk
   if (err <= 1.0 || H <= P.Hmin) {
     S.st[3] += 1;
     if (P.clip_negative)
-      #pragma unroll 4
-      for (int i = 0; i < N; ++i) Y[i] = max_f(Ynew[i], 0.0);
+      vbatch<VB>(N, [&](int i) -> double { return max_f(Ynew[i], 0.0); },
+                 [&](int i, double v) { Y[i] = v; });
     else
-      #pragma unroll 4
-      for (int i = 0; i < N; ++i) Y[i] = Ynew[i];
+      vbatch<VB>(N, [&](int i) -> double { return Ynew[i]; }, [&](int i, double v) { Y[i] = v; });
     S.T = S.T + dir * H;
     S.Hnew = max_f(P.Hmin, min_f(S.Hnew, P.Hmax));
     if (S.rejLast) S.Hnew = min_f(S.Hnew, H);
@@ -582,11 +818,11 @@ __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, double *blk
 // The carbon mechanism's whole integration (one forward-Euler step) on the
 // slot: the Euler kernel's algorithm.
 template <class M>
-__device__ __forceinline__ void lane_euler(const IntegrateArgs &a, double *blk, LaneState &S) {
+__device__ __forceinline__ void lane_euler(const IntegrateArgs &a, const Slot &sl, LaneState &S) {
   using L = SlotLayout<M>;
-  Col Y{blk + (int64_t)L::Y * 64, 64}, Vd{blk + (int64_t)L::F0 * 64, 64};
-  CCol Yc{Y.p, 64}, rct{blk + (int64_t)L::R * 64, 64};
-  dev_fun<M>(Yc, Yc, rct, Col{blk + (int64_t)L::S * 64, 64}, Vd);
+  const BCol Y = sl.col(L::Y), Vd = sl.col(L::F0), rct = sl.col(L::R);
+  const BCol &Yc = Y;
+  dev_fun<M>(Yc, Yc, rct, sl.col(L::S), Vd);
   const double dt = a.prm.Tend - a.prm.Tstart;
   bool neg = false;
   #pragma unroll 4
@@ -605,15 +841,16 @@ __device__ __forceinline__ void lane_euler(const IntegrateArgs &a, double *blk, 
 }
 
 template <class M>
-__device__ __forceinline__ void lane_finish(const IntegrateArgs &a, const double *blk,
-                                            int64_t cell, const LaneState &S) {
+__device__ __forceinline__ void lane_finish(const IntegrateArgs &a, const Slot &sl, int64_t cell,
+                                            const LaneState &S) {
   using L = SlotLayout<M>;
+  const BCol Y = sl.col(L::Y);
   // the integrator changes only the variable species; the step's write-back
   // clips every species (the callers' MAX(C,0) over NSPEC)
   const int nout = a.clip_out ? M::NSPEC : M::NVAR;
   #pragma unroll 4
   for (int s = 0; s < nout; ++s) {
-    double v = blk[(int64_t)(L::Y + s) * 64];
+    double v = Y[s];
     if (a.clip_out) v = v > 0.0 ? v : 0.0;
     a.conc[(int64_t)s * a.cstride + cell] = v;
   }
@@ -649,11 +886,14 @@ template <> struct IsEuler<Traits_carbon> : std::true_type {};
 // allow (1: unconstrained).
 // A workgroup holds blockDim.x/64 waves (cells per workgroup = blockDim.x);
 // each wave has its own slot block.
-template <int U, int W, class M0, class M1, class M2>
+template <int U, int W, bool SEG, class M0, class M1, class M2>
 __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  double *blk = pa.work + wave * (64 * pa.slot_fields) + lane;
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const Slot sl{__builtin_amdgcn_make_buffer_rsrc(pa.work + wave * (64 * pa.slot_fields), (short)0,
+                                                  (int)(64 * pa.slot_fields * 8), 0x00020000),
+                lane * 8};
   constexpr int NQ = std::is_void_v<M1> ? 1 : (std::is_void_v<M2> ? 2 : 3);
   const int64_t total = pa.qend[NQ - 1];
   int64_t cell = -1;
@@ -674,17 +914,17 @@ __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
       S.tries = 0;
       on_queue<M0, M1, M2>(q, [&](auto *tag) {
         using M = std::remove_pointer_t<decltype(tag)>;
-        lane_begin<M>(pa.q[q], blk, cell, false, S);
+        lane_begin<M>(pa.q[q], sl, cell, false, S);
       });
     }
     bool done = false;
     on_queue<M0, M1, M2>(q, [&](auto *tag) {
       using M = std::remove_pointer_t<decltype(tag)>;
       if constexpr (IsEuler<M>::value) {
-        lane_euler<M>(pa.q[q], blk, S);
+        lane_euler<M>(pa.q[q], sl, S);
         done = true;
       } else {
-        done = lane_attempt<M, U>(pa.q[q], blk, cell, S);
+        done = lane_attempt<M, U, SEG>(pa.q[q], sl, cell, S);
       }
     });
     if (done) {
@@ -692,13 +932,13 @@ __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
       if (S.ierr < 0 && a.retry && S.tries == 0) {
         on_queue<M0, M1, M2>(q, [&](auto *tag) {
           using M = std::remove_pointer_t<decltype(tag)>;
-          lane_begin<M>(a, blk, cell, true, S);
+          lane_begin<M>(a, sl, cell, true, S);
         });
         S.tries = 1;
       } else {
         on_queue<M0, M1, M2>(q, [&](auto *tag) {
           using M = std::remove_pointer_t<decltype(tag)>;
-          lane_finish<M>(a, blk, cell, S);
+          lane_finish<M>(a, sl, cell, S);
         });
         cell = -1;
       }
@@ -808,33 +1048,39 @@ void launch_integrate_mech(int mech, const IntegrateArgs &a, hipStream_t s) {
 
 // Persistent-launch variants: queue sets (one mechanism, or fullchem, Hg
 // and carbon in that order with any queue possibly empty) x U x W.
-template <int U, int W>
+template <int U, int W, bool SEG>
 static const void *persist_kernel(int qset) {
   switch (qset) {
-    case PQ_FULLCHEM: return (const void *)k_persist<U, W, Traits_fullchem, void, void>;
-    case PQ_HG: return (const void *)k_persist<U, W, Traits_Hg, void, void>;
-    case PQ_CARBON: return (const void *)k_persist<U, W, Traits_carbon, void, void>;
-    case PQ_MIXED: return (const void *)k_persist<U, W, Traits_fullchem, Traits_Hg, Traits_carbon>;
+    case PQ_FULLCHEM: return (const void *)k_persist<U, W, SEG, Traits_fullchem, void, void>;
+    case PQ_HG: return (const void *)k_persist<U, W, SEG, Traits_Hg, void, void>;
+    case PQ_CARBON: return (const void *)k_persist<U, W, SEG, Traits_carbon, void, void>;
+    case PQ_MIXED:
+      return (const void *)k_persist<U, W, SEG, Traits_fullchem, Traits_Hg, Traits_carbon>;
   }
   return nullptr;
 }
+// (U, W, SEG): terms per batched gather, minimum waves per SIMD (register
+// budget), flat segment execution of the independent sums
 const void *persist_kernel_variant(int qset, int variant) {
   switch (variant) {
-    case 0: return persist_kernel<1, 1>(qset);
-    case 1: return persist_kernel<4, 1>(qset);
-    case 2: return persist_kernel<8, 1>(qset);
-    case 3: return persist_kernel<4, 4>(qset);
-    case 4: return persist_kernel<8, 4>(qset);
+    case 0: return persist_kernel<1, 1, false>(qset);
+    case 1: return persist_kernel<8, 1, false>(qset);
+    case 2: return persist_kernel<8, 4, false>(qset);
+    case 3: return persist_kernel<8, 4, true>(qset);
+    case 4: return persist_kernel<16, 4, false>(qset);
+    case 5: return persist_kernel<16, 4, true>(qset);
+    case 6: return persist_kernel<8, 5, false>(qset);
   }
   return nullptr;
 }
 int persist_variant_info(int variant, int *U, int *W) {
-  static const int tab[][2] = {{1, 1}, {4, 1}, {8, 1}, {4, 4}, {8, 4}};
+  static const int tab[][2] = {{1, 1}, {8, 1}, {8, 4}, {8, 4}, {16, 4}, {16, 4}, {8, 5}};
   if (variant < 0 || variant >= PERSIST_VARIANTS) return -1;
   *U = tab[variant][0];
   *W = tab[variant][1];
   return 0;
 }
+int persist_variant_seg(int variant) { return variant == 3 || variant == 5; }
 int64_t persist_slot_fields(int qset) {
   switch (qset) {
     case PQ_FULLCHEM: return SlotLayout<Traits_fullchem>::FIELDS;

@@ -21,6 +21,7 @@ SYMBOLS = [
     "gckpp_integrate", "gckpp_fullchem_step_dev", "gckpp_fun_dev", "gckpp_jac_dev",
     "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_step_timings", "gckpp_set_solver",
     "gckpp_set_persistent_config", "gckpp_get_persistent_config", "gckpp_autotune_dev",
+    "gckpp_persistent_variants",
     "gckpp_batch_init", "gckpp_batch_set_config", "gckpp_batch_step_dev",
     "gckpp_batch_capture", "gckpp_batch_replay", "gckpp_batch_finalize",
     "gckpp_set_cell_order", "gckpp_set_step_log",
@@ -31,7 +32,7 @@ SYMBOLS = [
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
-PERSIST_VARIANTS = 5
+PERSIST_VARIANTS = 7        # gckpp_persistent_variants(); checked by the CPU tests
 
 
 class Part(ctypes.Structure):
@@ -81,6 +82,7 @@ def load():
     L.gckpp_solve_dev.argtypes = [vp, i64, dp, dp, vp]
     L.gckpp_step_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.gckpp_set_solver.argtypes = [vp, ctypes.c_int]
+    L.gckpp_persistent_variants.argtypes = []
     L.gckpp_set_persistent_config.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.gckpp_get_persistent_config.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
     L.gckpp_autotune_dev.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double, i32p, dp, dp, dp,
@@ -185,15 +187,17 @@ class Context:
 
     def persistent_config(self):
         """dict(variant, waves_per_group, waves, loads_in_flight, min_waves_per_simd)."""
-        out = (ctypes.c_int32 * 5)()
+        out = (ctypes.c_int32 * 6)()
         _check(self.L.gckpp_get_persistent_config(self.h, out), "gckpp_get_persistent_config")
         return dict(zip(("variant", "waves_per_group", "waves", "loads_in_flight",
-                         "min_waves_per_simd"), list(out)))
+                         "min_waves_per_simd", "segments"), list(out)))
 
     def autotune(self, conc, rconst, hstart, tin, tout, icntrl, rcntrl):
         """Times every persistent configuration on these cells (conc is not
         modified), keeps the fastest; returns ms[variant][log2 waves_per_group]."""
-        ms = (ctypes.c_float * (3 * PERSIST_VARIANTS))()
+        nv = self.L.gckpp_persistent_variants()
+        assert nv == PERSIST_VARIANTS, "library and binding disagree on the variant count"
+        ms = (ctypes.c_float * (3 * nv))()
         ic = np.ascontiguousarray(icntrl, dtype=np.int32)
         rc = np.ascontiguousarray(rcntrl, dtype=np.float64)
         _check(self.L.gckpp_autotune_dev(self.h, conc.shape[1], float(tin), float(tout),

@@ -16,6 +16,8 @@ wavefront-uniform index, so the compiler reads it through the scalar cache.
                         lower entry e (row-major): a = JVS(le_kk)/JVS(le_dj);
                         JVS(le_kk)=a; then JVS(lp_dst) -= a*JVS(lp_src) for
                         lp in [le_p0, le_p1)
+  cu_*, sf_*, sb_*      the same sums as flat segment lists (see build_tables)
+                        for the persistent solver's batched execution
 Run:  python emit_tables.py <mech> [<mech> ...]
 """
 import json
@@ -139,7 +141,53 @@ def build_tables(m):
             cr_ptr.append(len(cr_l))
             cr_div.append(diag[c] if c < k else -1)
     assert len(cr_l) == len(lp_dst)
-    return dict(rx_s=rx_s, fv_ptr=fv_ptr, fv_r=fv_r, fv_c=fv_c, pt_r=pt_r, pt_m=pt_m,
+    # flat segmented forms for latency-batched execution: each segment's
+    # first term carries its initial value with l index LU_NONZERO, the
+    # solver's constant entry -lu[LU_NONZERO] = 1 (see the persistent slot)
+    one = m["lu_nonzero"]
+    #  cu_*: the U part of each Crout row (positions q >= diag(k)), which are
+    #        independent once the row's L part is done: segment = position
+    cu_row, cu_sp, cu_q, cu_l, cu_u = [], [0], [], [], []
+    for k in range(nvar if crow else 0):
+        cu_row.append(len(cu_q))
+        for q in range(diag[k], crow[k + 1]):
+            cu_l.append(one)
+            cu_u.append(q)
+            for t in range(cr_ptr[q], cr_ptr[q + 1]):
+                cu_l.append(cr_l[t])
+                cu_u.append(cr_u[t])
+            cu_q.append(q)
+            cu_sp.append(len(cu_l))
+    cu_row.append(len(cu_q))
+    #  sf_*/sb_*: forward / backward substitution, segment = row, in
+    #  batches of consecutive rows that do not read each other's results
+    def solve_tables(order, lower):
+        sp, tl, tx, batch = [0], [], [], [0]
+        cur = set()
+        for n_, i in enumerate(order):
+            lo, hi = (crow[i], diag[i]) if lower else (diag[i] + 1, crow[i + 1])
+            deps = {lu_icol[k] for k in range(lo, hi)}
+            if cur & deps:
+                batch.append(n_)
+                cur = set()
+            cur.add(i)
+            tl.append(one)
+            tx.append(i)
+            for k in range(lo, hi):
+                tl.append(k)
+                tx.append(lu_icol[k])
+            sp.append(len(tl))
+        batch.append(len(order))
+        return sp, tl, tx, batch
+    if crow:
+        sf_sp, sf_l, sf_x, sf_batch = solve_tables(list(range(nvar)), True)
+        sb_sp, sb_l, sb_x, sb_batch = solve_tables(list(range(nvar - 1, -1, -1)), False)
+    else:
+        sf_sp = sf_l = sf_x = sf_batch = sb_sp = sb_l = sb_x = sb_batch = []
+    return dict(cu_row=cu_row, cu_sp=cu_sp, cu_q=cu_q, cu_l=cu_l, cu_u=cu_u,
+                sf_sp=sf_sp, sf_l=sf_l, sf_x=sf_x, sf_batch=sf_batch,
+                sb_sp=sb_sp, sb_l=sb_l, sb_x=sb_x, sb_batch=sb_batch,
+                rx_s=rx_s, fv_ptr=fv_ptr, fv_r=fv_r, fv_c=fv_c, pt_r=pt_r, pt_m=pt_m,
                 pt_s=pt_s, jv_ptr=jv_ptr, jv_b=jv_b, jv_c=jv_c, le_kk=le_kk, le_dj=le_dj,
                 le_p0=le_p0, le_p1=le_p1, le_row=le_row, lp_dst=lp_dst, lp_src=lp_src,
                 row_e0=row_e0, cr_ptr=cr_ptr, cr_l=cr_l, cr_u=cr_u, cr_div=cr_div)
@@ -155,10 +203,13 @@ def emit(name):
            % (m["nvar"], m["nfix"], m["nspec"], m["nreact"]),
            "constexpr int LU_NZ = %d, NPART = %d, RX_MAX = %d;" % (m["lu_nonzero"], len(t["pt_r"]), RX_MAX),
            "constexpr int NFV = %d, NJV = %d, NLE = %d, NLP = %d;"
-           % (len(t["fv_r"]), len(t["jv_b"]), len(t["le_kk"]), len(t["lp_dst"])), ""]
+           % (len(t["fv_r"]), len(t["jv_b"]), len(t["le_kk"]), len(t["lp_dst"])),
+           "constexpr int NSF_BATCH = %d, NSB_BATCH = %d;"
+           % (max(0, len(t["sf_batch"]) - 1), max(0, len(t["sb_batch"]) - 1)), ""]
     ints = ["rx_s", "fv_ptr", "fv_r", "pt_r", "pt_s", "jv_ptr", "jv_b", "le_kk", "le_dj",
             "le_p0", "le_p1", "le_row", "lp_dst", "lp_src", "row_e0", "cr_ptr", "cr_l", "cr_u",
-            "cr_div"]
+            "cr_div", "cu_row", "cu_sp", "cu_q", "cu_l", "cu_u", "sf_sp", "sf_l", "sf_x",
+            "sf_batch", "sb_sp", "sb_l", "sb_x", "sb_batch"]
     for k in ints:
         out.append(arr("int", k, t[k]))
     for k in ("fv_c", "pt_m", "jv_c"):

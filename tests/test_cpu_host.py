@@ -146,3 +146,91 @@ def test_fortran_binding_compiles_and_links(tmp_path):
                     "-Wl,-rpath," + str(libdir)], check=True, cwd=tmp_path)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
     assert [int(v) for v in out[:6]] == [0, 356, 353, 3, 1058, 5683]
+
+
+def test_flat_segment_tables_reproduce_lu_and_solve(fullchem):
+    """The flat segment tables of the persistent solver (Crout U part,
+    batched forward/backward substitution) give bit-identical results to the
+    position-by-position Crout factorisation and the row-by-row KppSolve
+    order on a random matrix with the LU pattern; batches hold only rows that
+    do not read each other."""
+    import sys
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "geos-chem_amd" / "mechgen"))
+    from emit_tables import build_tables
+    m = fullchem
+    raw = {"nvar": m.nvar, "nfix": m.nfix, "nreact": m.nreact, "lu_nonzero": m.lu_nonzero,
+           "reactions": m.reactions, "lu_irow": list(m.lu_irow), "lu_icol": list(m.lu_icol),
+           "lu_crow": list(m.lu_crow), "lu_diag": list(m.lu_diag)}
+    t = build_tables(raw)
+    nv, crow, diag, icol = m.nvar, m.lu_crow, m.lu_diag, m.lu_icol
+    rng = np.random.default_rng(0)
+    A = rng.uniform(-1, 1, m.lu_nonzero)
+    A[diag[:nv]] += 40.0
+    # reference: Crout position by position
+    ref = A.copy()
+    for k in range(nv):
+        for q in range(crow[k], crow[k + 1]):
+            acc = ref[q]
+            for tt in range(t["cr_ptr"][q], t["cr_ptr"][q + 1]):
+                acc = acc - ref[t["cr_l"][tt]] * ref[t["cr_u"][tt]]
+            if t["cr_div"][q] >= 0:
+                acc = acc / ref[t["cr_div"][q]]
+            ref[q] = acc
+    # flat: L part per position, U part by segments
+    lu = A.copy()
+    for k in range(nv):
+        for q in range(crow[k], diag[k]):
+            acc = lu[q]
+            for tt in range(t["cr_ptr"][q], t["cr_ptr"][q + 1]):
+                acc = acc - lu[t["cr_l"][tt]] * lu[t["cr_u"][tt]]
+            lu[q] = acc / lu[t["cr_div"][q]]
+        out = {}
+        for sg in range(t["cu_row"][k], t["cu_row"][k + 1]):
+            acc = 0.0
+            for tt in range(t["cu_sp"][sg], t["cu_sp"][sg + 1]):
+                w = 1.0 if t["cu_l"][tt] == m.lu_nonzero else -lu[t["cu_l"][tt]]
+                acc = acc + w * lu[t["cu_u"][tt]]
+            out[t["cu_q"][sg]] = acc
+        for q, v in out.items():
+            lu[q] = v
+    assert np.array_equal(lu, ref)
+    # solves
+    b = rng.normal(size=nv)
+    x = b.copy()
+    for i in range(nv):
+        acc = x[i]
+        for k in range(crow[i], diag[i]):
+            acc = acc - lu[k] * x[icol[k]]
+        x[i] = acc
+    for i in range(nv - 1, -1, -1):
+        acc = x[i]
+        for k in range(diag[i] + 1, crow[i + 1]):
+            acc = acc - lu[k] * x[icol[k]]
+        x[i] = acc / lu[diag[i]]
+    y = b.copy()
+    for sp, tl, tx, batch, back in ((t["sf_sp"], t["sf_l"], t["sf_x"], t["sf_batch"], False),
+                                    (t["sb_sp"], t["sb_l"], t["sb_x"], t["sb_batch"], True)):
+        for bb in range(len(batch) - 1):
+            res = {}
+            for sg in range(batch[bb], batch[bb + 1]):
+                acc, row = 0.0, None
+                for tt in range(sp[sg], sp[sg + 1]):
+                    if tl[tt] == m.lu_nonzero:
+                        row = tx[tt]
+                        acc = acc + 1.0 * y[tx[tt]]
+                    else:
+                        acc = acc - lu[tl[tt]] * y[tx[tt]]
+                        # a batch never reads a row it writes
+                        assert tx[tt] not in res
+                res[row] = acc
+            for row, v in res.items():
+                y[row] = v / lu[diag[row]] if back else v
+    assert np.array_equal(y, x)
+    assert len(t["sf_batch"]) - 1 < nv // 4 and len(t["sb_batch"]) - 1 < nv // 4
+
+
+def test_persistent_variant_count_matches_library():
+    from gckpp_amd import hip
+    if not hip.lib_path().exists():
+        pytest.skip("HIP library not built")
+    assert hip.load().gckpp_persistent_variants() == hip.PERSIST_VARIANTS

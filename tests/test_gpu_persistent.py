@@ -55,7 +55,7 @@ def box2k(fullchem):
     return dict(w=w, rct=rct, ic=ic, rc=rc, c=c_o, ist=i_o, ierr=e_o)
 
 
-@pytest.mark.parametrize("variant,wpb", [(0, 1), (1, 2), (2, 1), (2, 4), (3, 1), (4, 2)])
+@pytest.mark.parametrize("variant,wpb", [(0, 1), (1, 2), (2, 1), (3, 1), (4, 4), (5, 2), (6, 1)])
 def test_persistent_variants_match_oracle(fullchem, torch_dev, box2k, variant, wpb):
     print("case: persistent variant %d, %d waves per workgroup, 2000 box cells" % (variant, wpb),
           flush=True)
