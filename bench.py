@@ -51,7 +51,11 @@ def parse():
                    help="cells of the same grid timed with the CPU reference (0: skip)")
     p.add_argument("--tight-sample", type=int, default=48,
                    help="cells checked against the RTOL 1e-8 oracle (0: skip)")
-    p.add_argument("--safety", type=float, default=0.5, help="factor on the predicted h0")
+    p.add_argument("--safety", type=float, default=1.0, help="factor on the predicted h0")
+    p.add_argument("--variant", type=int, default=-1,
+                   help="persistent kernel variant (default: autotuned on the workload)")
+    p.add_argument("--wpb", type=int, default=1, help="waves per workgroup with --variant")
+    p.add_argument("--no-warm", action="store_true", help="skip the warm-step legs")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device copy (reference for HBM byte counters)")
     return p.parse_args()
@@ -166,9 +170,13 @@ def grid_bench(args):
 
     # autotune the persistent solver on the workload itself (a sample smaller
     # than the resident lanes would time a different, latency-only regime)
-    rct = ctx.rates(met, photol, het, conc0)
-    tune_ms = ctx.autotune(conc0, rct, cold, 0.0, w.dt, ic, rc)
-    del rct
+    tune_ms = None
+    if args.variant >= 0:
+        ctx.set_persistent_config(args.variant, args.wpb)
+    else:
+        rct = ctx.rates(met, photol, het, conc0)
+        tune_ms = ctx.autotune(conc0, rct, cold, 0.0, w.dt, ic, rc)
+        del rct
     pcfg = ctx.persistent_config()
     log("autotuned persistent config %s" % pcfg)
 
@@ -210,9 +218,11 @@ def grid_bench(args):
         wall = time.perf_counter() - t0
         st = ist.cpu().numpy()
         legs[name] = dict(ms=wall / args.steps * 1e3, cps=n / (wall / args.steps), tim=tim,
-                          conc=conc.cpu().numpy(), st=st, ierr=ierr.cpu().numpy())
+                          conc=conc.cpu().numpy(), st=st, ierr=ierr.cpu().numpy(),
+                          hnew=hnew.clone())
     ctx.set_cell_order(None)
     L, P = legs["learned"], legs["plain"]
+    warm = None if args.no_warm else warm_legs(args, mech, w, ctx, model, P, ic, rc)
 
     # dominant kernel: the persistent integrate of the learned leg
     nacc, nstp = L["st"][3].astype(np.int64), L["st"][2].astype(np.int64)
@@ -269,11 +279,78 @@ def grid_bench(args):
             "plain": {"mean_attempts": float(P["st"][2].mean()), "mean_rejects": float(P["st"][4].mean()),
                       "failed_cells": int((P["ierr"] < 0).sum()),
                       "integrate_ms": round(float(np.mean([t[1] for t in P["tim"]])), 3)}},
+        "warm_step": warm,
         "accuracy": accuracy,
-        "autotune_ms": [[round(float(v), 2) for v in row] for row in tune_ms],
+        "autotune_ms": None if tune_ms is None else [[round(float(v), 2) for v in row] for row in tune_ms],
         "cpu_baseline": base,
     }
     ctx.close()
+    return out
+
+
+def warm_legs(args, mech, w, ctx, model, P, ic, rc):
+    """The step after the benchmark step: the plain leg's result, perturbed by
+    a synthetic emission/transport operator (gckpp_amd.synth.perturb), with
+    the sun 20 minutes later.  Reference semantics: Hstart = KPPHvalue (the
+    plain leg's RSTATUS(3)), plain solver; learned: the MLP's h0 from the state
+    and KPPHvalue, cost order, persistent solver."""
+    import numpy as np
+    import torch
+    from gckpp_amd import hip, synth
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    w1 = synth.grid_cells(mech, nx, ny, nz, seed=1, hour_utc=12.0 + w.dt / 3600.0)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    met, photol, het = dev(w1.met), dev(w1.photol), dev(w1.het)
+    conc0 = dev(synth.perturb(mech, P["conc"], w1.met, seed=1))
+    kpph = P["hnew"]
+    n = w1.ncell
+    conc = conc0.clone()
+    ist = torch.empty((20, n), dtype=torch.int32, device="cuda")
+    ierr = torch.empty(n, dtype=torch.int32, device="cuda")
+    hnew = torch.empty(n, dtype=torch.float64, device="cuda")
+    h0 = torch.empty(n, dtype=torch.float64, device="cuda")
+    cls = torch.empty(n, dtype=torch.int32, device="cuda")
+
+    def plain():
+        conc.copy_(conc0)
+        ctx.set_solver(hip.SOLVER_PLAIN)
+        ctx.set_cell_order(None)
+        ctx.fullchem_step(met, photol, het, conc, kpph, w.dt, ic, rc, hnew=hnew, istatus=ist, ierr=ierr)
+
+    def learned():
+        conc.copy_(conc0)
+        ctx.set_solver(hip.SOLVER_PERSISTENT)
+        hs = kpph
+        if model is not None:
+            model.predict(met, photol, conc, hprev=kpph, dt=w.dt, h0=h0, cls=cls)
+            ctx.set_cell_order(hip.order_by_class(cls))
+            hs = h0
+        ctx.fullchem_step(met, photol, het, conc, hs, w.dt, ic, rc, hnew=hnew, istatus=ist, ierr=ierr)
+
+    out = {}
+    res = {}
+    for name, fn in (("plain_kpphvalue", plain), ("learned", learned)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        st = ist.cpu().numpy()
+        res[name] = conc.cpu().numpy()
+        out[name] = {"value": round(n / ms * 1e3, 2), "unit": "cells/s", "ms_per_step": round(ms, 3),
+                     "mean_attempts": float(st[2].mean()), "failed_cells": int((ierr < 0).sum().item())}
+        log("warm step %s: %.1f ms, %.2f attempts/cell" % (name, ms, st[2].mean()))
+    ctx.set_cell_order(None)
+    out["speedup_learned_vs_plain"] = round(out["plain_kpphvalue"]["ms_per_step"] /
+                                            out["learned"]["ms_per_step"], 3)
+    out["attempts_saved_frac"] = round(1.0 - out["learned"]["mean_attempts"] /
+                                       out["plain_kpphvalue"]["mean_attempts"], 4)
+    out["learned_vs_plain"] = rel_errors(mech, res["learned"], res["plain_kpphvalue"])
+    out["workload"] = ("the step after the timed step: its plain-leg result perturbed by synthetic "
+                       "emissions (NO, CO, ISOP, CH2O, SO2, ALD2 in P > 700 hPa) and 5% transport "
+                       "noise, sun 20 min later, Hstart = KPPHvalue (reference) or learned h0")
     return out
 
 

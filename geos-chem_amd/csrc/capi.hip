@@ -81,7 +81,7 @@ extern "C" int gckpp_mech_sizes(int mech_id, int32_t out[8]) {
 
 // Persistent configuration and its launch size: every resident wave, at
 // most one wave per 64 cells of the workspace, a whole number of workgroups.
-static const int kDefaultPersistVariant = 2, kDefaultPersistWpb = 1;
+static const int kDefaultPersistVariant = 4, kDefaultPersistWpb = 1;
 static int set_pconfig(gckpp_ctx *c, int variant, int wpb) {
   int64_t cap = c->wstride / 64 / wpb * wpb;
   int64_t res = persist_resident_waves(c->mech, variant, wpb);  // PQ_* == MECH_* for one mechanism
@@ -961,19 +961,19 @@ extern "C" int gckpp_mlp_get_params(gckpp_mlp *m, float *params) {
 }
 
 extern "C" int gckpp_mlp_features_dev(gckpp_mlp *m, int64_t ncell, const double *met,
-                                      const double *photol, const double *conc, float *X,
-                                      void *stream) {
+                                      const double *photol, const double *conc,
+                                      const double *hprev, float *X, void *stream) {
   if (!m || !met || !photol || !conc || !X) return fail("gckpp_mlp_features: null argument");
   if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_mlp_features: negative ncell");
-  mlp_features(m->desc, ncell, met, photol, conc, X, (hipStream_t)stream);
+  mlp_features(m->desc, ncell, met, photol, conc, hprev, X, (hipStream_t)stream);
   HIP_OK(hipGetLastError());
   return 0;
 }
 
 extern "C" int gckpp_mlp_predict_dev(gckpp_mlp *m, int64_t ncell, const double *met,
-                                     const double *photol, const double *conc, double safety,
-                                     double hmin, double hmax, double *h0, int32_t *cls,
-                                     void *stream) {
+                                     const double *photol, const double *conc,
+                                     const double *hprev, double safety, double hmin,
+                                     double hmax, double *h0, int32_t *cls, void *stream) {
   if (!m || !met || !photol || !conc) return fail("gckpp_mlp_predict: null argument");
   if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_mlp_predict: negative ncell");
   if (!(hmin > 0.0) || !(hmax >= hmin) || !(safety > 0.0))
@@ -983,7 +983,8 @@ extern "C" int gckpp_mlp_predict_dev(gckpp_mlp *m, int64_t ncell, const double *
     mlp_pack(m->params, m->packed, s);
     m->packed_ok = true;
   }
-  mlp_predict(m->desc, ncell, met, photol, conc, m->packed, m->params, safety, hmin, hmax, h0, cls, s);
+  mlp_predict(m->desc, ncell, met, photol, conc, hprev, m->packed, m->params, safety, hmin, hmax,
+              h0, cls, s);
   HIP_OK(hipGetLastError());
   return 0;
 }

@@ -8,7 +8,7 @@ namespace gckpp {
 
 constexpr int MLP_NJ = 12;                  // photolysis channels used
 constexpr int MLP_NS = 24;                  // species used
-constexpr int MLP_NIN = 4 + MLP_NJ + MLP_NS;  // 40 inputs
+constexpr int MLP_NIN = 4 + MLP_NJ + MLP_NS + 2;  // 42 inputs
 constexpr int MLP_F = 48;                   // inputs padded to the MFMA K step
 constexpr int MLP_H = 128;                  // hidden units per layer (3 layers)
 constexpr int MLP_O = 16;                   // outputs padded (log h0 + 4 logits used)
@@ -27,7 +27,9 @@ constexpr int MLP_NPACK = (MLP_F * MLP_H + 2 * MLP_H * MLP_H + MLP_H * MLP_O);
 
 // Inputs: feature k = (x_k - mean[k]) * rstd[k] of
 //   0 TEMP, 1 log10 PRESS, 2 log10 NUMDEN, 3 SUNCOS,
-//   4..15 log10(PHOTOL(jslot)+1e-20), 16..39 log10(C(spc)+1); 40..47 zero.
+//   4..15 log10(PHOTOL(jslot)+1e-20), 16..39 log10(C(spc)+1),
+//   40 ln(hprev) and 41 = 1 when the cell has a previous step size hprev > 0
+//   (KPPHvalue), else 0 and 0; 42..47 zero.
 // Output 0 is (ln h0 - out_mu) / out_sigma.
 struct MlpDesc {
   int32_t jslot[MLP_NJ];
@@ -38,10 +40,11 @@ struct MlpDesc {
 
 void mlp_pack(const float *params, unsigned short *pk, hipStream_t s);
 void mlp_features(const MlpDesc &d, int64_t n, const double *met, const double *photol,
-                  const double *conc, float *X, hipStream_t s);
+                  const double *conc, const double *hprev, float *X, hipStream_t s);
 void mlp_predict(const MlpDesc &d, int64_t n, const double *met, const double *photol,
-                 const double *conc, const unsigned short *pk, const float *params, double safety,
-                 double hmin, double hmax, double *h0, int32_t *cls, hipStream_t s);
+                 const double *conc, const double *hprev, const unsigned short *pk,
+                 const float *params, double safety, double hmin, double hmax, double *h0,
+                 int32_t *cls, hipStream_t s);
 void mlp_forward_f32(const float *P, int64_t nb, const float *X, float *act, hipStream_t s);
 void mlp_grad_f32(const float *P, int64_t nb, const float *X, const float *yh, const int32_t *ycls,
                   float wh, float wc, float *act, float *dO, float *dA, float *G, float *lossp,

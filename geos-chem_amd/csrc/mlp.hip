@@ -41,14 +41,18 @@ __device__ __forceinline__ float gelu_grad(float z) {
 // Feature f of a cell, normalised.
 __device__ __forceinline__ float mlp_feature(int f, const MlpDesc &d, const double *met,
                                              const double *photol, const double *conc,
-                                             int64_t n, int64_t cell) {
+                                             const double *hprev, int64_t n, int64_t cell) {
   double v;
+  const double hp = hprev ? hprev[cell] : 0.0;
   if (f == 0) v = met[(int64_t)MET_TEMP * n + cell];
   else if (f == 1) v = log10(fmax(met[(int64_t)MET_PRESS * n + cell], 1e-30));
   else if (f == 2) v = log10(fmax(met[(int64_t)MET_NUMDEN * n + cell], 1e-30));
   else if (f == 3) v = met[(int64_t)MET_SUNCOS * n + cell];
   else if (f < 4 + MLP_NJ) v = log10(fmax(photol[(int64_t)d.jslot[f - 4] * n + cell], 0.0) + 1e-20);
-  else if (f < MLP_NIN) v = log10(fmax(conc[(int64_t)d.spc[f - 4 - MLP_NJ] * n + cell], 0.0) + 1.0);
+  else if (f < 4 + MLP_NJ + MLP_NS)
+    v = log10(fmax(conc[(int64_t)d.spc[f - 4 - MLP_NJ] * n + cell], 0.0) + 1.0);
+  else if (f == MLP_NIN - 2) v = hp > 0.0 ? log(hp) : 0.0;
+  else if (f == MLP_NIN - 1) v = hp > 0.0 ? 1.0 : 0.0;
   else return 0.0f;
   return (float)((v - d.mean[f]) * d.rstd[f]);
 }
@@ -56,12 +60,12 @@ __device__ __forceinline__ float mlp_feature(int f, const MlpDesc &d, const doub
 // X[cell][MLP_F] (fp32, padded features zero).
 __global__ __launch_bounds__(256) void k_mlp_features(int64_t n, MlpDesc d, const double *met,
                                                       const double *photol, const double *conc,
-                                                      float *X) {
+                                                      const double *hprev, float *X) {
   int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n * MLP_F) return;
   int64_t cell = i / MLP_F;
   int f = (int)(i % MLP_F);
-  X[i] = mlp_feature(f, d, met, photol, conc, n, cell);
+  X[i] = mlp_feature(f, d, met, photol, conc, hprev, n, cell);
 }
 
 // Packs W[in][out] (fp32, row-major) of one layer into MFMA A-operand tiles of
@@ -82,7 +86,8 @@ __global__ void k_mlp_pack(const float *W, int nin, int nout, unsigned short *pk
 // 4(l/16)..4(l/16)+3 of each 16-row tile.
 __global__ __launch_bounds__(256) void k_mlp_predict(int64_t n, MlpDesc d, const double *met,
                                                      const double *photol, const double *conc,
-                                                     const bf16x4 *pk, const float *params,
+                                                     const double *hprev, const bf16x4 *pk,
+                                                     const float *params,
                                                      double safety, double hmin, double hmax,
                                                      double *h0, int32_t *cls) {
   const int l = threadIdx.x & 63;
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(256) void k_mlp_predict(int64_t n, MlpDesc d, const
   for (int c = 0; c < MLP_F / 16; ++c)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      b[c][j] = (short)to_bf16(mlp_feature(16 * c + 4 * g + j, d, met, photol, conc, n, cc));
+      b[c][j] = (short)to_bf16(mlp_feature(16 * c + 4 * g + j, d, met, photol, conc, hprev, n, cc));
   const bf16x4 *pk1 = pk;
   const bf16x4 *pk2 = pk1 + (MLP_H / 16) * (MLP_F / 16) * 64;
   const bf16x4 *pk3 = pk2 + (MLP_H / 16) * (MLP_H / 16) * 64;
@@ -291,17 +296,18 @@ void mlp_pack(const float *params, unsigned short *pk, hipStream_t s) {
 }
 
 void mlp_features(const MlpDesc &d, int64_t n, const double *met, const double *photol,
-                  const double *conc, float *X, hipStream_t s) {
+                  const double *conc, const double *hprev, float *X, hipStream_t s) {
   hipLaunchKernelGGL(k_mlp_features, dim3(nblk(n * MLP_F, 256)), dim3(256), 0, s, n, d, met, photol,
-                     conc, X);
+                     conc, hprev, X);
 }
 
 void mlp_predict(const MlpDesc &d, int64_t n, const double *met, const double *photol,
-                 const double *conc, const unsigned short *pk, const float *params, double safety,
-                 double hmin, double hmax, double *h0, int32_t *cls, hipStream_t s) {
+                 const double *conc, const double *hprev, const unsigned short *pk,
+                 const float *params, double safety, double hmin, double hmax, double *h0,
+                 int32_t *cls, hipStream_t s) {
   int64_t tiles = (n + 15) / 16;
   hipLaunchKernelGGL(k_mlp_predict, dim3(nblk(tiles, 4)), dim3(256), 0, s, n, d, met, photol, conc,
-                     (const bf16x4 *)pk, params, safety, hmin, hmax, h0, cls);
+                     hprev, (const bf16x4 *)pk, params, safety, hmin, hmax, h0, cls);
 }
 
 // fp32 forward of nb rows; act holds Z1,A1,Z2,A2,Z3,A3 ([nb][128] each) and O [nb][16].

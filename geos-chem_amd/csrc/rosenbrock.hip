@@ -232,10 +232,11 @@ __device__ __forceinline__ void vbatch(int n, F f, ST st) {
 
 
 // A(r) = RCT(r)*prod(reactants) into a[], then Vdot(i) = sum c*A(r).
+// With U > 1 the operands of 4 reactions are loaded before their products
+// (same products, same order); with SEG the Vdot sums go as flat segments.
 template <class M, int U = 1, bool SEG = false, class CY, class CF, class CR, class CA, class CV>
 __device__ void dev_fun(const CY &y, const CF &fix, const CR &rct, CA a, CV vdot) {
-  if constexpr (SEG) {
-    // operands of 4 reactions loaded, then their products (same order)
+  if constexpr (U > 1) {
     vbatch<4>(
         M::NREACT,
         [&](int r) -> double {
@@ -252,37 +253,37 @@ __device__ void dev_fun(const CY &y, const CF &fix, const CR &rct, CA a, CV vdot
           return v;
         },
         [&](int r, double v) { a[r] = v; });
-    seg_gather<(U < 8 ? U : 8)>(0, M::NVAR, [&](int i) { return M::fv_ptr(i); },
-                  [&](int t) -> double { return M::fv_c(t); },
-                  [&](int t) -> double { return a[M::fv_r(t)]; },
-                  [&](int i, double acc) { vdot[i] = acc; });
-    return;
-  }
+  } else {
 #pragma unroll 2
-  for (int r = 0; r < M::NREACT; ++r) {
-    double v = rct[r];
+    for (int r = 0; r < M::NREACT; ++r) {
+      double v = rct[r];
 #pragma unroll
-    for (int q = 0; q < M::RX_MAX; ++q) {
-      int s = M::rx_s(r * M::RX_MAX + q);
-      if (s < 0) break;
-      v *= spc<M>(y, fix, s);
+      for (int q = 0; q < M::RX_MAX; ++q) {
+        int s = M::rx_s(r * M::RX_MAX + q);
+        if (s < 0) break;
+        v *= spc<M>(y, fix, s);
+      }
+      a[r] = v;
     }
-    a[r] = v;
   }
+  if constexpr (SEG) {
+    seg_gather<(U < 8 ? U : 8)>(0, M::NVAR, [&](int i) { return M::fv_ptr(i); },
+                                [&](int t) -> double { return M::fv_c(t); },
+                                [&](int t) -> double { return a[M::fv_r(t)]; },
+                                [&](int i, double acc) { vdot[i] = acc; });
+  } else {
 #pragma unroll 1
-  for (int i = 0; i < M::NVAR; ++i)
-    vdot[i] = gather_fma<U>(M::fv_ptr(i), M::fv_ptr(i + 1), 0.0,
-                            [&](int t) -> double { return M::fv_c(t); },
-                            [&](int t) -> double { return a[M::fv_r(t)]; });
+    for (int i = 0; i < M::NVAR; ++i)
+      vdot[i] = gather_fma<U>(M::fv_ptr(i), M::fv_ptr(i + 1), 0.0,
+                              [&](int t) -> double { return M::fv_c(t); },
+                              [&](int t) -> double { return a[M::fv_r(t)]; });
+  }
 }
 
-// JVS in LU layout: partials into b[], then each LU entry gathers its terms;
-// out[k] = sign*JVS(k) (+ diag on the diagonal), so sign=-1, diag=1/(H*gamma)
-// builds ros_PrepareMatrix's Ghimj directly.
-template <class M, int U = 1, bool SEG = false, class CY, class CF, class CR, class CB, class CO>
-__device__ void dev_jac(const CY &y, const CF &fix, const CR &rct, CB b, CO out, double sign,
-                        double diag) {
-  if constexpr (SEG) {
+// The partial derivatives b[p] = d A(pt_r)/d V = pt_m*RCT*prod(other reactants).
+template <class M, int U, class CY, class CF, class CR, class CB>
+__device__ __forceinline__ void dev_partials(const CY &y, const CF &fix, const CR &rct, CB b) {
+  if constexpr (U > 1) {
     vbatch<4>(
         M::NPART,
         [&](int p) -> double {
@@ -299,36 +300,86 @@ __device__ void dev_jac(const CY &y, const CF &fix, const CR &rct, CB b, CO out,
           return v;
         },
         [&](int p, double v) { b[p] = v; });
+  } else {
+#pragma unroll 2
+    for (int p = 0; p < M::NPART; ++p) {
+      double v = M::pt_m(p) * rct[M::pt_r(p)];
+#pragma unroll
+      for (int q = 0; q < M::RX_MAX - 1; ++q) {
+        int s = M::pt_s(p * (M::RX_MAX - 1) + q);
+        if (s < 0) break;
+        v *= spc<M>(y, fix, s);
+      }
+      b[p] = v;
+    }
+  }
+}
+
+// JVS in LU layout: partials into b[], then each LU entry gathers its terms;
+// out[k] = sign*JVS(k) (+ diag on the diagonal), so sign=-1, diag=1/(H*gamma)
+// builds ros_PrepareMatrix's Ghimj directly.
+template <class M, int U = 1, bool SEG = false, class CY, class CF, class CR, class CB, class CO>
+__device__ void dev_jac(const CY &y, const CF &fix, const CR &rct, CB b, CO out, double sign,
+                        double diag) {
+  dev_partials<M, U>(y, fix, rct, b);
+  if constexpr (SEG) {
     seg_gather<(U < 8 ? U : 8)>(0, M::LU_NZ, [&](int k) { return M::jv_ptr(k); },
-                  [&](int t) -> double { return M::jv_c(t); },
-                  [&](int t) -> double { return b[M::jv_b(t)]; },
-                  [&](int k, double acc) { out[k] = sign * acc; });
-    if (diag != 0.0)
+                                [&](int t) -> double { return M::jv_c(t); },
+                                [&](int t) -> double { return b[M::jv_b(t)]; },
+                                [&](int k, double acc) { out[k] = sign * acc; });
+  } else {
+#pragma unroll 1
+    for (int k = 0; k < M::LU_NZ; ++k) {
+      double acc = gather_fma<U>(M::jv_ptr(k), M::jv_ptr(k + 1), 0.0,
+                                 [&](int t) -> double { return M::jv_c(t); },
+                                 [&](int t) -> double { return b[M::jv_b(t)]; });
+      out[k] = sign * acc;
+    }
+  }
+  if (diag != 0.0) {
+    if constexpr (U > 1)
       vbatch<8>(M::NVAR, [&](int i) -> double { return (double)out[M::lu_diag(i)] + diag; },
                 [&](int i, double v) { out[M::lu_diag(i)] = v; });
-    return;
-  }
-#pragma unroll 2
-  for (int p = 0; p < M::NPART; ++p) {
-    double v = M::pt_m(p) * rct[M::pt_r(p)];
-#pragma unroll
-    for (int q = 0; q < M::RX_MAX - 1; ++q) {
-      int s = M::pt_s(p * (M::RX_MAX - 1) + q);
-      if (s < 0) break;
-      v *= spc<M>(y, fix, s);
-    }
-    b[p] = v;
-  }
-#pragma unroll 1
-  for (int k = 0; k < M::LU_NZ; ++k) {
-    double acc = gather_fma<U>(M::jv_ptr(k), M::jv_ptr(k + 1), 0.0,
-                               [&](int t) -> double { return M::jv_c(t); },
-                               [&](int t) -> double { return b[M::jv_b(t)]; });
-    out[k] = sign * acc;
-  }
-  if (diag != 0.0)
+    else
 #pragma unroll 4
-    for (int i = 0; i < M::NVAR; ++i) out[M::lu_diag(i)] += diag;
+      for (int i = 0; i < M::NVAR; ++i) out[M::lu_diag(i)] += diag;
+  }
+}
+
+// ros_PrepareMatrix's Ghimj = sign*JVS + diag (on the diagonal) and its
+// KppDecomp in one pass: each LU position starts from its Jacobian sum
+// (gathered from the partials) instead of a stored Ghimj, then takes the
+// row-Crout updates.  The same values as dev_jac followed by
+// dev_decomp_crout, without writing and re-reading Ghimj.  Returns IER.
+template <class M, int U, class CY, class CF, class CR, class CB, class CL>
+__device__ int dev_jac_decomp(const CY &y, const CF &fix, const CR &rct, CB b, CL lu, double sign,
+                              double diag) {
+  dev_partials<M, U>(y, fix, rct, b);
+  auto ghimj = [&](int q) -> double {
+    return sign * gather_fma<U>(M::jv_ptr(q), M::jv_ptr(q + 1), 0.0,
+                                [&](int t) -> double { return M::jv_c(t); },
+                                [&](int t) -> double { return b[M::jv_b(t)]; });
+  };
+  int ier = 0;
+  int q = 0;
+#pragma unroll 1
+  for (int k = 0; k < M::NVAR; ++k) {
+    const int kd = M::lu_diag(k);
+    const double gd = ghimj(kd) + diag;  // Ghimj(k,k) before row k's elimination
+    if (ier == 0 && fabs(gd) < DBL_MIN) ier = k + 1;
+    const int q1 = M::lu_crow(k + 1);
+#pragma unroll 1
+    for (; q < q1; ++q) {
+      const double g = q == kd ? gd : ghimj(q);
+      double acc = gather_fma<U>(M::cr_ptr(q), M::cr_ptr(q + 1), g,
+                                 [&](int t) -> double { return -(double)lu[M::cr_l(t)]; },
+                                 [&](int t) -> double { return lu[M::cr_u(t)]; });
+      const int dv = M::cr_div(q);
+      if (dv >= 0) acc = acc / lu[dv];
+      lu[q] = acc;
+    }
+  }
+  return ier;
 }
 
 // KppDecomp in place; returns IER (0 or the 1-based row whose diagonal was
@@ -676,7 +727,7 @@ __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, const Slot 
                                              LaneState &S) {
   using L = SlotLayout<M>;
   const int N = M::NVAR;
-  constexpr int VB = SEG ? 4 : 1;
+  constexpr int VB = U > 1 ? 4 : 1;
   const BCol Y = sl.col(L::Y), Fcn0 = sl.col(L::F0), Ynew = sl.col(L::YN), Ascr = sl.col(L::S),
              LU = sl.col(L::LU), rct = sl.col(L::R);
   const BCol &Yc = Y, &Ynewc = Ynew;
@@ -698,8 +749,13 @@ __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, const Slot 
   int nconsec = 0;
   bool singular = true;
   while (singular) {
-    dev_jac<M, U, SEG>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * S.H * kRosGamma1));
-    int ising = dev_decomp_crout<M, U, SEG>(LU);
+    int ising;
+    if constexpr (SEG) {
+      dev_jac<M, U, SEG>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * S.H * kRosGamma1));
+      ising = dev_decomp_crout<M, U, SEG>(LU);
+    } else {
+      ising = dev_jac_decomp<M, U>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * S.H * kRosGamma1));
+    }
     S.st[5] += 1;
     if (ising == 0) {
       singular = false;

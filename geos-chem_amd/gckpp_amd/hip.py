@@ -100,8 +100,8 @@ def load():
     L.gckpp_mlp_set_norm.argtypes = [vp, fp, fp, ctypes.c_float, ctypes.c_float]
     L.gckpp_mlp_set_params.argtypes = [vp, fp]
     L.gckpp_mlp_get_params.argtypes = [vp, fp]
-    L.gckpp_mlp_features_dev.argtypes = [vp, i64, dp, dp, dp, vp, vp]
-    L.gckpp_mlp_predict_dev.argtypes = [vp, i64, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+    L.gckpp_mlp_features_dev.argtypes = [vp, i64, dp, dp, dp, dp, vp, vp]
+    L.gckpp_mlp_predict_dev.argtypes = [vp, i64, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, dp, vp, vp]
     L.gckpp_mlp_forward_dev.argtypes = [vp, i64, vp, vp, vp]
     L.gckpp_mlp_grad_dev.argtypes = [vp, i64, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp, vp, vp]
@@ -424,14 +424,16 @@ class MLP:
                "gckpp_mlp_get_params")
         return out
 
-    def features(self, met, photol, conc):
+    def features(self, met, photol, conc, hprev=None):
         n = conc.shape[1]
         X = self.torch.empty((n, 48), dtype=self.torch.float32, device=conc.device)
-        _check(self.L.gckpp_mlp_features_dev(self.h, n, _ptr(met), _ptr(photol), _ptr(conc), _ptr(X),
-                                             _stream()), "gckpp_mlp_features_dev")
+        _check(self.L.gckpp_mlp_features_dev(self.h, n, _ptr(met), _ptr(photol), _ptr(conc),
+                                             _ptr(hprev), _ptr(X), _stream()),
+               "gckpp_mlp_features_dev")
         return X
 
-    def predict(self, met, photol, conc, safety=1.0, hmin=1e-5, hmax=1200.0, h0=None, cls=None):
+    def predict(self, met, photol, conc, hprev=None, safety=1.0, hmin=1e-5, hmax=1200.0, h0=None,
+                cls=None):
         T = self.torch
         n = conc.shape[1]
         if h0 is None:
@@ -439,8 +441,8 @@ class MLP:
         if cls is None:
             cls = T.empty((n,), dtype=T.int32, device=conc.device)
         _check(self.L.gckpp_mlp_predict_dev(self.h, n, _ptr(met), _ptr(photol), _ptr(conc),
-                                            float(safety), float(hmin), float(hmax), _ptr(h0),
-                                            _ptr(cls), _stream()), "gckpp_mlp_predict_dev")
+                                            _ptr(hprev), float(safety), float(hmin), float(hmax),
+                                            _ptr(h0), _ptr(cls), _stream()), "gckpp_mlp_predict_dev")
         return h0, cls
 
     def forward(self, X):

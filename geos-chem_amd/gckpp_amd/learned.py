@@ -1,23 +1,24 @@
 """Learned initial step size and stiffness class for the Rosenbrock solve.
 
-A cold chemistry step starts every cell at Hstart = 1e-5 s
-(KPP/fullchem/gckpp_Integrator.F90:389-395, RCNTRL(3)=0) and the step-size
-controller needs several accepted steps, each at most FacMax=6 times the
-last, before the error estimate limits the step.  The MLP (csrc/mlp.hip)
-predicts, from the cell's state, the step size the controller settles on
-and a cost class; the solve then starts each cell at the predicted size and
-the persistent solver takes the costliest cells first.  The controller is
-unchanged: it accepts or rejects every step exactly as before, so results
-meet the same tolerance.
+Each cell's integration starts at Hstart: 1e-5 s on a cold step
+(KPP/fullchem/gckpp_Integrator.F90:389-395, RCNTRL(3)=0), else the previous
+step's last proposed step, KPPHvalue = RSTATUS(3)
+(GeosCore/fullchem_mod.F90:1016-1020,1294).  After the operators that run
+between chemistry steps (emissions, transport) a cell's state has moved and
+that step can be far off, costing rejected attempts; on a cold step the
+controller needs several accepted steps before the error limits the step.
+The MLP (csrc/mlp.hip) predicts, from the cell's state, the step size the
+controller settles on and a cost class; the solve then starts each cell at
+the predicted size and the persistent solver takes the costliest cells
+first.  The controller is unchanged: it accepts or rejects every step
+exactly as before, so results meet the same tolerance.
 
 Training data are the step histories the GPU Rosenbrock logs
-(gckpp_set_step_log): (H, err) of each cell's first attempts.  Label of a
-cell: at its first attempt whose error limits the step
-(err > (FacSafe/FacMax)^3), the step the controller proposes,
-H*min(FacMax, max(FacMin, FacSafe/err^(1/3))) (gckpp_Integrator.F90:725-735);
-if no logged attempt is error-limited, the last H times FacMax, at most dt.
-Class: quartile of the cell's number of attempts (ISTATUS(3)) in the
-training set, 0 cheapest .. 3 costliest.
+(gckpp_set_step_log): (H, err) of each cell's first attempts, on cold steps
+of synthetic grids and on the warm steps that follow them after a synthetic
+emission/transport perturbation (gckpp_amd.synth.perturb).  Labels: see
+labels().  Class: quartile of the cell's number of attempts (ISTATUS(3)) in
+the training set, 0 cheapest .. 3 costliest.
 """
 import json
 from pathlib import Path
@@ -29,6 +30,7 @@ J_SLOTS = [1, 2, 10, 8, 9, 6, 7, 15, 14, 11, 13, 60]
 SPECIES = ["O3", "NO", "NO2", "OH", "HO2", "CO", "CH4", "ISOP", "HNO3", "H2O2", "CH2O", "NO3",
            "N2O5", "PAN", "SO2", "BrO", "Cl", "ClO", "MP", "ALD2", "ACET", "C2H6", "MO2", "HNO2"]
 NLOG = 24                       # attempts logged per cell
+NIN = 42                        # model inputs (csrc/mlp.h)
 FAC_MIN, FAC_MAX, FAC_SAFE = 0.2, 6.0, 0.9
 MODEL_DIR = Path(__file__).resolve().parent.parent / "models"
 
@@ -41,21 +43,29 @@ def inputs(mech):
 
 def labels(hlog, nstep, dt):
     """ln(h) label per cell from a step log hlog[2*NLOG][ncell] (float) and
-    the number of attempts per cell nstep[ncell] (ISTATUS(3))."""
+    the number of attempts per cell nstep[ncell] (ISTATUS(3)).  If the first
+    attempt was rejected (err > 1): the step of the first accepted attempt.
+    Otherwise: at the first error-limited attempt (err > (FacSafe/FacMax)^3),
+    the step the controller proposes; if none is logged, the last H times
+    FacMax, at most dt."""
     nlog = hlog.shape[0] // 2
     H = hlog[0::2].astype(np.float64)
     E = hlog[1::2].astype(np.float64)
     n = hlog.shape[1]
+    cols = np.arange(n)
     valid = np.arange(nlog)[:, None] < np.minimum(nstep, nlog)[None, :]
     limited = valid & (E > (FAC_SAFE / FAC_MAX) ** 3)
     first = np.where(limited.any(axis=0), limited.argmax(axis=0), -1)
-    cols = np.arange(n)
     k = np.maximum(first, 0)
     fac = np.minimum(FAC_MAX, np.maximum(FAC_MIN, FAC_SAFE / np.maximum(E[k, cols], 1e-300) ** (1 / 3)))
     h_lim = H[k, cols] * fac
     last = np.maximum(np.minimum(nstep, nlog) - 1, 0)
     h_free = np.minimum(H[last, cols] * FAC_MAX, dt)
     h = np.where(first >= 0, h_lim, h_free)
+    accepted = valid & (E <= 1.0)
+    first_acc = np.where(accepted.any(axis=0), accepted.argmax(axis=0), 0)
+    rejected0 = valid[0] & (E[0] > 1.0)
+    h = np.where(rejected0, H[first_acc, cols], h)
     return np.log(np.clip(h, 1e-12, dt))
 
 
@@ -96,18 +106,18 @@ class StepModel:
         return self
 
     # -- data -----------------------------------------------------------------
-    def raw_features(self, met, photol, conc):
+    def raw_features(self, met, photol, conc, hprev=None):
         """[ncell][48] un-normalised inputs, computed by the features kernel."""
         self.net.set_norm(np.zeros(48, np.float32), np.ones(48, np.float32), 0.0, 1.0)
-        X = self.net.features(met, photol, conc)
+        X = self.net.features(met, photol, conc, hprev)
         self.net.set_norm(self.mean, self.std, self.out_mu, self.out_sigma)
         return X
 
     def fit_normalisation(self, Xraw, lnh, nstep):
         """Input mean/std, label mean/std and class thresholds from training data."""
-        x = Xraw[:, :40].double()
-        self.mean[:40] = x.mean(dim=0).cpu().numpy()
-        self.std[:40] = np.maximum(x.std(dim=0).cpu().numpy(), 1e-6)
+        x = Xraw[:, :NIN].double()
+        self.mean[:NIN] = x.mean(dim=0).cpu().numpy()
+        self.std[:NIN] = np.maximum(x.std(dim=0).cpu().numpy(), 1e-6)
         self.out_mu, self.out_sigma = float(np.mean(lnh)), float(max(np.std(lnh), 1e-6))
         self.thresholds = [float(v) for v in np.quantile(nstep, [0.25, 0.5, 0.75])]
         self.net.set_norm(self.mean, self.std, self.out_mu, self.out_sigma)
@@ -117,7 +127,7 @@ class StepModel:
         m = torch.from_numpy(self.mean).to(Xraw.device)
         s = torch.from_numpy(self.std).to(Xraw.device)
         X = (Xraw - m) / s
-        X[:, 40:] = 0.0
+        X[:, NIN:] = 0.0
         return X.contiguous()
 
     # -- training -------------------------------------------------------------
@@ -153,13 +163,18 @@ class StepModel:
         return hist
 
     # -- use --------------------------------------------------------------------
-    def predict(self, met, photol, conc, safety=0.5, dt=1200.0, h0=None, cls=None):
-        return self.net.predict(met, photol, conc, safety=safety, hmin=1e-5, hmax=dt, h0=h0, cls=cls)
+    def predict(self, met, photol, conc, hprev=None, safety=1.0, dt=1200.0, h0=None, cls=None):
+        """Hstart per cell (hprev: the previous step size per cell, None on a
+        cold step) and the cost class."""
+        return self.net.predict(met, photol, conc, hprev=hprev, safety=safety, hmin=1e-5, hmax=dt,
+                                h0=h0, cls=cls)
 
 
-def log_step_histories(ctx, w_dev, ic, rc, dt, nlog=NLOG):
-    """Cold-start persistent step of one workload with the step log on:
-    returns (hlog [2*nlog][n] float32 host, nstep [n] host, ierr)."""
+def log_step_histories(ctx, w_dev, ic, rc, dt, nlog=NLOG, hstart=None):
+    """One persistent chemistry step of a workload with the step log on,
+    from Hstart = hstart (device tensor; None: cold start).  Returns (hlog
+    [2*nlog][n] float32 host, nstep [n] host, ierr host, conc after the step,
+    RSTATUS(3)) -- the last two device tensors."""
     import torch
     from . import hip
     n = w_dev["conc"].shape[1]
@@ -167,11 +182,21 @@ def log_step_histories(ctx, w_dev, ic, rc, dt, nlog=NLOG):
     ctx.set_solver(hip.SOLVER_PERSISTENT)
     ctx.set_step_log(hlog, nlog)
     conc = w_dev["conc"].clone()
-    hs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    hs = torch.zeros(n, dtype=torch.float64, device="cuda") if hstart is None else hstart
     hnew, ist, ierr, ok = ctx.fullchem_step(w_dev["met"], w_dev["photol"], w_dev["het"], conc, hs,
                                             dt, ic, rc)
     ctx.set_step_log(None)
-    return hlog.cpu().numpy(), ist[2].cpu().numpy(), ierr.cpu().numpy()
+    return hlog.cpu().numpy(), ist[2].cpu().numpy(), ierr.cpu().numpy(), conc, hnew
+
+
+def warm_pair(mech, nx, ny, nz, seed, hour, dt=1200.0):
+    """A grid for a warm chemistry step: the workload at `hour` (its cold
+    step gives the state and KPPHvalue), and the met/photolysis 20 minutes
+    later that the next step sees; the caller perturbs the state between."""
+    from . import synth
+    w0 = synth.grid_cells(mech, nx, ny, nz, seed=seed, hour_utc=hour, dt=dt)
+    w1 = synth.grid_cells(mech, nx, ny, nz, seed=seed, hour_utc=(hour + dt / 3600.0) % 24.0, dt=dt)
+    return w0, w1
 
 
 def default_model_path(mech_name="fullchem"):

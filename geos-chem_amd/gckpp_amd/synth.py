@@ -236,6 +236,31 @@ def grid_cells(mech, nx=72, ny=46, nz=72, seed=1, dt=1200.0, nspec_fields=None, 
                                                "" if hour_utc == 12.0 else "_%gUTC" % hour_utc))
 
 
+# Emission rates [molec/cm3/s] of the boundary-layer cells (P > 700 hPa),
+# log-uniform between the bounds: a synthetic stand-in for the emission and
+# transport operators that run between two chemistry steps.
+_EMIS = {"NO": (1e4, 3e6), "CO": (1e5, 3e7), "ISOP": (1e3, 3e6), "CH2O": (1e3, 1e6),
+         "SO2": (1e3, 1e6), "ALD2": (1e3, 3e5)}
+
+
+def perturb(mech, conc, met, dt=1200.0, seed=0, mixing=0.05):
+    """The concentrations after the operators between two chemistry steps:
+    every variable species scaled by exp(N(0, mixing)) (transport), then the
+    emissions of _EMIS over dt added in the boundary layer.  Returns a new
+    array."""
+    rng = np.random.default_rng(seed)
+    n = conc.shape[1]
+    out = np.array(conc, copy=True)
+    nv = mech.nvar
+    out[:nv] *= np.exp(rng.normal(0.0, mixing, (nv, n)))
+    bl = met[fields.MET["PRESS"][0]] > 700.0
+    for name, (lo, hi) in _EMIS.items():
+        if name in mech.species:
+            e = 10.0 ** rng.uniform(np.log10(lo), np.log10(hi), n)
+            out[mech.species.index(name)] += np.where(bl, e * dt, 0.0)
+    return out
+
+
 def mixed_grid(mechs, nx=144, ny=91, nz=72, seed=3, split=(0.7, 0.2, 0.1), hour_utc=12.0):
     """Cells of one nx*ny*nz grid assigned at random to the mechanisms mechs
     (e.g. fullchem, Hg, carbon) with probabilities split; returns one

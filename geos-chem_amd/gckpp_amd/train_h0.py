@@ -1,14 +1,16 @@
 """Trains the step-size MLP on step histories logged by the GPU Rosenbrock
 solve of synthetic grids and writes geos-chem_amd/models/h0_mlp_<mech>.npz.
 
-Each logged step is a cold-start chemistry step (dt = 1200 s) of one
-72x46x72 synthetic grid (gckpp_amd.synth.grid_cells) with its own seed and
-UTC hour; seeds start at 100 so the benchmark grid (seed 1) is never seen.
-The persistent solver logs (H, err) of each cell's first attempts; labels
-and classes follow gckpp_amd.learned.
+Each synthetic grid (72x46x72, gckpp_amd.synth.grid_cells, its own seed and
+UTC hour; seeds start at 100 so the benchmark grid, seed 1, is never seen)
+gives two logged chemistry steps (dt = 1200 s): its cold step, and the warm
+step that follows a synthetic emission/transport perturbation
+(gckpp_amd.synth.perturb) with the sun 20 minutes later and Hstart =
+KPPHvalue from the cold step.  --grids 24 gives 48 logged steps (~11 M
+samples).  The persistent solver logs (H, err) of each cell's first
+attempts; labels and classes follow gckpp_amd.learned.
 
-Run on a GPU:  python -m gckpp_amd.train_h0 [--steps 48] [--epochs 4]
-(from geos-chem_amd/, or with it on sys.path).
+Run on a GPU:  python geos-chem_amd/gckpp_amd/train_h0.py [--grids 24] [--epochs 4]
 """
 import argparse
 import sys
@@ -25,7 +27,7 @@ sys.path.insert(0, str(HERE.parent.parent))
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--mech", default="fullchem")
-    ap.add_argument("--steps", type=int, default=48, help="logged grid steps")
+    ap.add_argument("--grids", type=int, default=24, help="synthetic grids (2 logged steps each)")
     ap.add_argument("--grid", default="72x46x72")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--batch", type=int, default=8192)
@@ -44,18 +46,23 @@ def main():
     dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
     Xs, lhs, nss = [], [], []
     t0 = time.time()
-    for k in range(a.steps):
-        seed, hour = 100 + k, (k * 24.0 / a.steps + 0.5 * (k % 2)) % 24.0
-        w = synth.grid_cells(mech, nx, ny, nz, seed=seed, hour_utc=hour)
-        d = dict(met=dev(w.met), photol=dev(w.photol), het=dev(w.het), conc=dev(w.conc))
-        hlog, nstep, ierr = learned.log_step_histories(ctx, d, ic, rc, w.dt)
-        ok = ierr == 1
-        Xs.append(model.raw_features(d["met"], d["photol"], d["conc"])[torch.from_numpy(ok).cuda()])
-        lhs.append(learned.labels(hlog, nstep, w.dt)[ok])
-        nss.append(nstep[ok])
-        print("logged step %d/%d: seed %d, %.1f UTC, %d cells, mean attempts %.1f (%.0f s)"
-              % (k + 1, a.steps, seed, hour, int(ok.sum()), nstep.mean(), time.time() - t0), flush=True)
-        del d
+    for k in range(a.grids):
+        seed, hour = 100 + k, (k * 24.0 / a.grids + 0.5 * (k % 2)) % 24.0
+        w0, w1 = learned.warm_pair(mech, nx, ny, nz, seed, hour)
+        d0 = dict(met=dev(w0.met), photol=dev(w0.photol), het=dev(w0.het), conc=dev(w0.conc))
+        hlog0, n0, e0, c1, h1 = learned.log_step_histories(ctx, d0, ic, rc, w0.dt)
+        c1p = synth.perturb(mech, c1.cpu().numpy(), w1.met, seed=seed)
+        d1 = dict(met=dev(w1.met), photol=dev(w1.photol), het=dev(w1.het), conc=dev(c1p))
+        hlog1, n1, e1, _, _ = learned.log_step_histories(ctx, d1, ic, rc, w0.dt, hstart=h1.clone())
+        for d, hlog, nstep, ierr, hprev in ((d0, hlog0, n0, e0, None), (d1, hlog1, n1, e1, h1)):
+            ok = ierr == 1
+            X = model.raw_features(d["met"], d["photol"], d["conc"], hprev)
+            Xs.append(X[torch.from_numpy(ok).cuda()])
+            lhs.append(learned.labels(hlog, nstep, w0.dt)[ok])
+            nss.append(nstep[ok])
+        print("grid %d/%d: seed %d, %.1f UTC: cold step %.1f attempts/cell, warm step %.1f (%.0f s)"
+              % (k + 1, a.grids, seed, hour, n0.mean(), n1.mean(), time.time() - t0), flush=True)
+        del d0, d1
     Xraw = torch.cat(Xs)
     lnh, nstep = np.concatenate(lhs), np.concatenate(nss)
     del Xs

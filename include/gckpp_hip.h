@@ -191,13 +191,15 @@ int gckpp_batch_capture(gckpp_batch *b, const gckpp_part parts[3], double dt);
 int gckpp_batch_replay(gckpp_batch *b, void *stream);
 int gckpp_batch_finalize(gckpp_batch *b);
 
-/* Step-size MLP (no reference counterpart; it replaces the cold-start
- * Hstart of 1e-5 s, gckpp_Integrator.F90:389-395, by a prediction, and the
+/* Step-size MLP (no reference counterpart; it replaces Hstart -- the cold
+ * start's 1e-5 s, gckpp_Integrator.F90:389-395, or the previous step's
+ * KPPHvalue, GeosCore/fullchem_mod.F90:1016-1020 -- by a prediction, and the
  * integrator's step control still accepts or rejects every step).
- * 40 inputs: T, log10 P, log10 NUMDEN, SUNCOS, log10(J+1e-20) of the 12
+ * 42 inputs: T, log10 P, log10 NUMDEN, SUNCOS, log10(J+1e-20) of the 12
  * PHOTOL slots jslot[] (0-based), log10(C+1) of the 24 species spc[]
- * (0-based KPP indices); normalised by mean/std (48 entries, the last 8
- * unused).  3 hidden layers of 128 GELU units; outputs (ln h0 - out_mu)/
+ * (0-based KPP indices), ln(hprev) and 1 for cells with a previous step
+ * size hprev > 0 (hprev[ncell] may be null: cold, 0 and 0); normalised by
+ * mean/std (48 entries, the last 6 unused).  3 hidden layers of 128 GELU units; outputs (ln h0 - out_mu)/
  * out_sigma and 4 stiffness-class logits.  Parameters: GCKPP_MLP_NPARAM
  * floats, per layer W[in][out] row-major then bias, layer sizes 48x128,
  * 128x128, 128x128, 128x16 (outputs 5..15 unused).
@@ -218,12 +220,12 @@ int gckpp_mlp_set_norm(gckpp_mlp *m, const float mean[48], const float stdev[48]
 int gckpp_mlp_set_params(gckpp_mlp *m, const float *params);
 int gckpp_mlp_get_params(gckpp_mlp *m, float *params);
 int gckpp_mlp_features_dev(gckpp_mlp *m, int64_t ncell, const double *met,
-                           const double *photol, const double *conc, float *X,
-                           void *stream);
+                           const double *photol, const double *conc,
+                           const double *hprev, float *X, void *stream);
 int gckpp_mlp_predict_dev(gckpp_mlp *m, int64_t ncell, const double *met,
-                          const double *photol, const double *conc, double safety,
-                          double hmin, double hmax, double *h0, int32_t *cls,
-                          void *stream);
+                          const double *photol, const double *conc,
+                          const double *hprev, double safety, double hmin,
+                          double hmax, double *h0, int32_t *cls, void *stream);
 int gckpp_mlp_forward_dev(gckpp_mlp *m, int64_t nb, const float *X, float *out,
                           void *stream);
 int gckpp_mlp_grad_dev(gckpp_mlp *m, int64_t nb, const float *X, const float *yh,

@@ -12,7 +12,7 @@ from scipy.special import erf
 
 F, H, O = 48, 128, 16
 NJ, NS = 12, 24
-NIN = 4 + NJ + NS
+NIN = 4 + NJ + NS + 2
 # parameter layout: W[in][out] row-major then bias, layer after layer
 SHAPES = [(F, H), (H, H), (H, H), (H, O)]
 OFFS = []
@@ -42,8 +42,9 @@ def gelu_grad(z):
     return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
 
 
-def raw_features(jslot, spc, met, photol, conc):
-    """[ncell][NIN] un-normalised inputs (layouts [field][ncell])."""
+def raw_features(jslot, spc, met, photol, conc, hprev=None):
+    """[ncell][NIN] un-normalised inputs (layouts [field][ncell]); the last
+    two: ln(hprev) and 1 where hprev > 0 (a previous step size), else 0."""
     n = conc.shape[1]
     x = np.zeros((n, NIN))
     x[:, 0] = met[MET_TEMP]
@@ -54,12 +55,16 @@ def raw_features(jslot, spc, met, photol, conc):
         x[:, 4 + k] = np.log10(np.maximum(photol[s], 0.0) + 1e-20)
     for k, s in enumerate(spc):
         x[:, 4 + NJ + k] = np.log10(np.maximum(conc[s], 0.0) + 1.0)
+    if hprev is not None:
+        warm = hprev > 0.0
+        x[:, NIN - 2] = np.where(warm, np.log(np.where(warm, hprev, 1.0)), 0.0)
+        x[:, NIN - 1] = warm
     return x
 
 
-def features(jslot, spc, mean, std, met, photol, conc):
+def features(jslot, spc, mean, std, met, photol, conc, hprev=None):
     """[ncell][F] normalised inputs, padded with zeros."""
-    x = raw_features(jslot, spc, met, photol, conc)
+    x = raw_features(jslot, spc, met, photol, conc, hprev)
     X = np.zeros((x.shape[0], F))
     X[:, :NIN] = (x - mean[:NIN]) * (1.0 / std[:NIN])
     return X

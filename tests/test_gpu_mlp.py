@@ -33,13 +33,14 @@ def setup(fullchem, torch_dev):
     rng = np.random.default_rng(1)
     mean = np.zeros(48, np.float32)
     std = np.ones(48, np.float32)
-    mean[:40] = rng.normal(0, 2, 40)
-    std[:40] = rng.uniform(0.5, 3.0, 40)
+    mean[:42] = rng.normal(0, 2, 42)
+    std[:42] = rng.uniform(0.5, 3.0, 42)
     net = hip.MLP(4096, jslot, spc)
     net.set_norm(mean, std, -2.0, 1.7)
     p = (rng.normal(0.0, 0.12, hip.MLP_NPARAM)).astype(np.float32)
     net.set_params(p)
-    yield dict(w=w, jslot=jslot, spc=spc, mean=mean, std=std, net=net, p=p)
+    hprev = np.where(rng.uniform(size=w.ncell) < 0.5, 0.0, 10.0 ** rng.uniform(-5, 3, w.ncell))
+    yield dict(w=w, jslot=jslot, spc=spc, mean=mean, std=std, net=net, p=p, hprev=hprev)
     net.close()
 
 
@@ -47,12 +48,18 @@ def test_features_match_numpy(setup, torch_dev):
     print("case: MLP features, 2048 box cells", flush=True)
     from oracle import mlp
     s, w = setup, setup["w"]
-    X = s["net"].features(_t(torch_dev, w.met), _t(torch_dev, w.photol), _t(torch_dev, w.conc))
+    X = s["net"].features(_t(torch_dev, w.met), _t(torch_dev, w.photol), _t(torch_dev, w.conc),
+                          _t(torch_dev, s["hprev"]))
     ref = mlp.features(s["jslot"], s["spc"], s["mean"].astype(np.float64), s["std"].astype(np.float64),
-                       w.met, w.photol, w.conc)
+                       w.met, w.photol, w.conc, s["hprev"])
     x = X.cpu().numpy()
     assert np.all(np.abs(x - ref) <= 1e-4 * np.abs(ref) + 1e-5)
-    assert np.all(x[:, 40:] == 0.0)
+    assert np.all(x[:, 42:] == 0.0)
+    # cold (no previous step size): the last two inputs are zero before normalisation
+    Xc = s["net"].features(_t(torch_dev, w.met), _t(torch_dev, w.photol), _t(torch_dev, w.conc))
+    refc = mlp.features(s["jslot"], s["spc"], s["mean"].astype(np.float64), s["std"].astype(np.float64),
+                        w.met, w.photol, w.conc)
+    assert np.all(np.abs(Xc.cpu().numpy() - refc) <= 1e-4 * np.abs(refc) + 1e-5)
 
 
 def test_forward_and_gradient_match_numpy(setup, torch_dev):
@@ -60,7 +67,7 @@ def test_forward_and_gradient_match_numpy(setup, torch_dev):
     from oracle import mlp
     rng = np.random.default_rng(2)
     X = rng.normal(size=(1024, 48)).astype(np.float32)
-    X[:, 40:] = 0
+    X[:, 42:] = 0
     yh = rng.normal(size=1024).astype(np.float32)
     yc = rng.integers(0, 4, 1024).astype(np.int32)
     net, p = setup["net"], setup["p"]
@@ -84,9 +91,9 @@ def test_bf16_mfma_inference_matches_emulation(setup, torch_dev):
     from oracle import mlp
     s, w = setup, setup["w"]
     h0, cls = s["net"].predict(_t(torch_dev, w.met), _t(torch_dev, w.photol), _t(torch_dev, w.conc),
-                               safety=1.0, hmin=1e-300, hmax=1e300)
+                               hprev=_t(torch_dev, s["hprev"]), safety=1.0, hmin=1e-300, hmax=1e300)
     X = mlp.features(s["jslot"], s["spc"], s["mean"].astype(np.float64), s["std"].astype(np.float64),
-                     w.met, w.photol, w.conc).astype(np.float32)
+                     w.met, w.photol, w.conc, s["hprev"]).astype(np.float32)
     o = mlp.forward_bf16(s["p"], X)
     lnh = np.log(h0.cpu().numpy())
     out0 = (lnh - (-2.0)) / 1.7
@@ -117,7 +124,7 @@ def test_training_reduces_loss(setup, torch_dev):
     T = torch_dev
     rng = np.random.default_rng(5)
     X = rng.normal(size=(32768, 48)).astype(np.float32)
-    X[:, 40:] = 0
+    X[:, 42:] = 0
     yh = (np.tanh(X[:, 0] - 0.5 * X[:, 3]) + 0.3 * X[:, 7]).astype(np.float32)
     yc = (np.digitize(X[:, 1] + X[:, 2], [-1.0, 0.0, 1.0])).astype(np.int32)
     net = hip.MLP(4096, setup["jslot"], setup["spc"])
@@ -138,62 +145,76 @@ def test_training_reduces_loss(setup, torch_dev):
 
 def test_learned_h0_saves_attempts_within_tolerance(fullchem, torch_dev):
     """Trained on the step histories the GPU solver logs on two synthetic
-    grids, the network's h0 and cost order cut the attempts per cell on a
-    third (held-out) grid, and every cell stays within the repo tolerance of
-    the cold-start solve."""
-    print("case: learned h0 gain, 36x23x24 grids", flush=True)
+    grids (a cold step each, then a warm step after a synthetic emission and
+    transport perturbation), the network's h0 cuts the attempts of the warm
+    step of a third (held-out) grid against the reference's Hstart =
+    KPPHvalue, never costs attempts on its cold step, and every cell stays
+    within the repo tolerance of the reference-Hstart solve."""
+    print("case: learned h0 gain, 36x23x24 grids, cold and warm steps", flush=True)
     from gckpp_amd import hip, learned, synth
     from oracle.ref import fullchem_icntrl
     T = torch_dev
     ic, rc = fullchem_icntrl()
-    ctx = hip.Context(fullchem, 36 * 23 * 24)
+    nx, ny, nz = 36, 23, 24
+    ctx = hip.Context(fullchem, nx * ny * nz)
     model = learned.StepModel(fullchem, max_batch=8192)
 
-    def dev(w):
-        return dict(met=_t(T, w.met), photol=_t(T, w.photol), het=_t(T, w.het), conc=_t(T, w.conc))
+    def dev(met, photol, het, conc):
+        return dict(met=_t(T, met), photol=_t(T, photol), het=_t(T, het), conc=_t(T, conc))
+
+    def pair(seed, hour):
+        w0, w1 = learned.warm_pair(fullchem, nx, ny, nz, seed, hour)
+        d0 = dev(w0.met, w0.photol, w0.het, w0.conc)
+        hlog0, n0, e0, c1, h1 = learned.log_step_histories(ctx, d0, ic, rc, w0.dt)
+        c1p = synth.perturb(fullchem, c1.cpu().numpy(), w1.met, seed=seed)
+        d1 = dev(w1.met, w1.photol, w1.het, c1p)
+        return w0, d0, hlog0, n0, e0, d1, h1
 
     Xs, lh, ns = [], [], []
     for seed, hour in ((21, 6.0), (22, 15.0)):
-        w = synth.grid_cells(fullchem, 36, 23, 24, seed=seed, hour_utc=hour)
-        d = dev(w)
-        hlog, nstep, ierr = learned.log_step_histories(ctx, d, ic, rc, w.dt)
-        assert (ierr == 1).all()
-        Xs.append(model.raw_features(d["met"], d["photol"], d["conc"]))
-        lh.append(learned.labels(hlog, nstep, w.dt))
-        ns.append(nstep)
+        w0, d0, hlog0, n0, e0, d1, h1 = pair(seed, hour)
+        hlog1, n1, e1, _, _ = learned.log_step_histories(ctx, d1, ic, rc, w0.dt, hstart=h1.clone())
+        assert (e0 == 1).all() and (e1 == 1).all()
+        Xs += [model.raw_features(d0["met"], d0["photol"], d0["conc"]),
+               model.raw_features(d1["met"], d1["photol"], d1["conc"], h1)]
+        lh += [learned.labels(hlog0, n0, w0.dt), learned.labels(hlog1, n1, w0.dt)]
+        ns += [n0, n1]
     Xraw = T.cat(Xs)
     lnh, nstep = np.concatenate(lh), np.concatenate(ns)
     model.fit_normalisation(Xraw, lnh, nstep)
     X = model.normalise(Xraw)
     yh = _t(T, (lnh - model.out_mu) / model.out_sigma, T.float32)
     yc = _t(T, learned.classes(nstep, model.thresholds), T.int32)
-    hist = model.train(X, yh, yc, epochs=6, batch=2048, lr=3e-3)
+    hist = model.train(X, yh, yc, epochs=8, batch=2048, lr=3e-3)
     print("training loss per epoch", hist)
     assert hist[-1] < hist[0]
     # held-out grid
-    w = synth.grid_cells(fullchem, 36, 23, 24, seed=23, hour_utc=10.0)
-    d = dev(w)
-    n = w.ncell
+    w0, d0, _, n_cold, _, d1, h1 = pair(23, 10.0)
     res = {}
-    for label in ("cold", "learned"):
+    for label, d, hs in (("warm_ref", d1, h1), ("warm_learned", d1, None),
+                         ("cold_ref", d0, T.zeros_like(h1)), ("cold_learned", d0, None)):
+        if hs is None:
+            hprev = h1 if label.startswith("warm") else None
+            hs, cls = model.predict(d["met"], d["photol"], d["conc"], hprev=hprev, dt=w0.dt)
+            ctx.set_cell_order(hip.order_by_class(cls))
+        else:
+            ctx.set_cell_order(None)
         ctx.set_solver(hip.SOLVER_PERSISTENT)
         conc = d["conc"].clone()
-        if label == "cold":
-            hs = T.zeros(n, dtype=T.float64, device="cuda")
-            ctx.set_cell_order(None)
-        else:
-            hs, cls = model.predict(d["met"], d["photol"], d["conc"], safety=0.5, dt=w.dt)
-            ctx.set_cell_order(hip.order_by_class(cls))
-        hnew, ist, ierr, ok = ctx.fullchem_step(d["met"], d["photol"], d["het"], conc, hs, w.dt, ic, rc)
+        hnew, ist, ierr, ok = ctx.fullchem_step(d["met"], d["photol"], d["het"], conc, hs.clone(),
+                                                w0.dt, ic, rc)
         assert ok
-        res[label] = (conc.cpu().numpy(), ist.cpu().numpy())
+        res[label] = (conc.cpu().numpy(), ist.cpu().numpy()[2].sum())
     ctx.set_cell_order(None)
-    a_cold, a_learn = res["cold"][1][2].sum(), res["learned"][1][2].sum()
-    print("attempts: cold %d learned %d (%.1f%% fewer)" % (a_cold, a_learn, 100 * (1 - a_learn / a_cold)))
-    assert a_learn < 0.9 * a_cold
-    nv = fullchem.nvar
-    c0, c1 = res["cold"][0], res["learned"][0]
-    err = np.abs(c1[:nv] - c0[:nv]) / (fullchem.atol()[:, None] + fullchem.rtol()[:, None] * np.abs(c0[:nv]))
-    print("max scaled difference learned vs cold", err.max())
-    assert err.max() <= 1.0
+    for kind in ("warm", "cold"):
+        a_ref, a_learn = res[kind + "_ref"][1], res[kind + "_learned"][1]
+        print("%s step attempts: reference Hstart %d, learned %d (%.1f%% fewer)"
+              % (kind, a_ref, a_learn, 100 * (1 - a_learn / a_ref)))
+        c0, c1 = res[kind + "_ref"][0], res[kind + "_learned"][0]
+        nv = fullchem.nvar
+        err = np.abs(c1[:nv] - c0[:nv]) / (fullchem.atol()[:, None] + fullchem.rtol()[:, None] * np.abs(c0[:nv]))
+        print("  max scaled difference learned vs reference Hstart", err.max())
+        assert err.max() <= 1.0
+    assert res["warm_learned"][1] < 0.97 * res["warm_ref"][1]
+    assert res["cold_learned"][1] <= 1.01 * res["cold_ref"][1]
     ctx.close()

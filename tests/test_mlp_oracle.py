@@ -10,7 +10,7 @@ def test_layout_matches_abi_size():
     from oracle import mlp
     from gckpp_amd import hip
     assert mlp.NPARAM == hip.MLP_NPARAM == 41360
-    assert mlp.NIN == 40 and mlp.F == 48
+    assert mlp.NIN == 42 and mlp.F == 48
 
 
 def test_gradient_matches_finite_differences():
