@@ -82,6 +82,11 @@ extern "C" int gckpp_mech_sizes(int mech_id, int32_t out[8]) {
 // Persistent configuration and its launch size: every resident wave, at
 // most one wave per 64 cells of the workspace, a whole number of workgroups.
 static const int kDefaultPersistVariant = 4, kDefaultPersistWpb = 1;
+// Waves a persistent launch needs for ncell cells (before the resident cap):
+// a lane-per-cell wave takes 64 cells at a time, a wave-per-cell wave one.
+static int64_t cells_to_waves(int variant, int64_t ncell) {
+  return variant == PERSIST_WAVE_VARIANT ? ncell : (ncell + 63) / 64;
+}
 static int set_pconfig(gckpp_ctx *c, int variant, int wpb) {
   int64_t cap = c->wstride / 64 / wpb * wpb;
   int64_t res = persist_resident_waves(c->mech, variant, wpb);  // PQ_* == MECH_* for one mechanism
@@ -318,7 +323,7 @@ static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
     pa.work = c->work;
     pa.slot_fields = persist_slot_fields(c->mech);
     pa.counter = c->counter;
-    int64_t nw = (ncell + 63) / 64;
+    int64_t nw = cells_to_waves(c->pvariant, ncell);
     nw = (nw + c->pwpb - 1) / c->pwpb * c->pwpb;
     if (nw > c->pwaves) nw = c->pwaves;
     if (launch_persist(c->mech, c->pvariant, c->pwpb, pa, nw, s) != 0)
@@ -546,7 +551,7 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
       pa.work = c->work;
       pa.slot_fields = persist_slot_fields(c->mech);
       pa.counter = c->counter;
-      int64_t nw = (ncell + 63) / 64;
+      int64_t nw = cells_to_waves(c->pvariant, ncell);
       nw = (nw + c->pwpb - 1) / c->pwpb * c->pwpb;
       if (nw > c->pwaves) nw = c->pwaves;
       if (launch_persist(c->mech, c->pvariant, c->pwpb, pa, nw, s) != 0)
@@ -813,7 +818,7 @@ static int batch_enqueue(gckpp_batch *b, const gckpp_part parts[3], double dt, h
   pa.work = b->work;
   pa.slot_fields = b->slot_fields;
   pa.counter = b->counter;
-  int64_t nw = (end + 63) / 64;
+  int64_t nw = cells_to_waves(b->variant, end);
   nw = (nw + b->wpb - 1) / b->wpb * b->wpb;
   if (nw > b->nwaves) nw = b->nwaves;
   if (launch_persist(PQ_MIXED, b->variant, b->wpb, pa, nw, s) != 0)

@@ -18,9 +18,7 @@
 #include <type_traits>
 
 #include "gckpp_internal.h"
-#include "gen/mech_Hg.h"
-#include "gen/mech_carbon.h"
-#include "gen/mech_fullchem.h"
+#include "mech_traits.h"
 
 namespace gckpp {
 
@@ -60,64 +58,6 @@ struct BCol {
 };
 
 // ---------------------------------------------------------------------------
-// Mechanism-generic device pieces.  M is one of the mech_<name> namespaces,
-// wrapped in a traits struct so it can be a template argument.
-#define GCKPP_MECH_TRAITS(NS)                                                  \
-  struct Traits_##NS {                                                         \
-    static constexpr int NVAR = mech_##NS::NVAR, NFIX = mech_##NS::NFIX;       \
-    static constexpr int NSPEC = mech_##NS::NSPEC, NREACT = mech_##NS::NREACT; \
-    static constexpr int LU_NZ = mech_##NS::LU_NZ, NPART = mech_##NS::NPART;   \
-    static constexpr int RX_MAX = mech_##NS::RX_MAX;                           \
-    static __device__ int rx_s(int i) { return mech_##NS::rx_s[i]; }           \
-    static __device__ int fv_ptr(int i) { return mech_##NS::fv_ptr[i]; }       \
-    static __device__ int fv_r(int i) { return mech_##NS::fv_r[i]; }           \
-    static __device__ double fv_c(int i) { return mech_##NS::fv_c[i]; }        \
-    static __device__ int pt_r(int i) { return mech_##NS::pt_r[i]; }           \
-    static __device__ double pt_m(int i) { return mech_##NS::pt_m[i]; }        \
-    static __device__ int pt_s(int i) { return mech_##NS::pt_s[i]; }           \
-    static __device__ int jv_ptr(int i) { return mech_##NS::jv_ptr[i]; }       \
-    static __device__ int jv_b(int i) { return mech_##NS::jv_b[i]; }           \
-    static __device__ double jv_c(int i) { return mech_##NS::jv_c[i]; }        \
-    static __device__ int row_e0(int i) { return mech_##NS::row_e0[i]; }       \
-    static __device__ int le_kk(int i) { return mech_##NS::le_kk[i]; }         \
-    static __device__ int le_dj(int i) { return mech_##NS::le_dj[i]; }         \
-    static __device__ int le_p0(int i) { return mech_##NS::le_p0[i]; }         \
-    static __device__ int le_p1(int i) { return mech_##NS::le_p1[i]; }         \
-    static __device__ int lp_dst(int i) { return mech_##NS::lp_dst[i]; }       \
-    static __device__ int lp_src(int i) { return mech_##NS::lp_src[i]; }       \
-    static __device__ int lu_crow(int i) { return mech_##NS::lu_crow[i]; }     \
-    static __device__ int lu_diag(int i) { return mech_##NS::lu_diag[i]; }     \
-    static __device__ int lu_icol(int i) { return mech_##NS::lu_icol[i]; }     \
-    static __device__ int cr_ptr(int i) { return mech_##NS::cr_ptr[i]; }       \
-    static __device__ int cr_l(int i) { return mech_##NS::cr_l[i]; }           \
-    static __device__ int cr_u(int i) { return mech_##NS::cr_u[i]; }           \
-    static __device__ int cr_div(int i) { return mech_##NS::cr_div[i]; }       \
-    static __device__ int cu_row(int i) { return mech_##NS::cu_row[i]; }       \
-    static __device__ int cu_sp(int i) { return mech_##NS::cu_sp[i]; }         \
-    static __device__ int cu_q(int i) { return mech_##NS::cu_q[i]; }           \
-    static __device__ int cu_l(int i) { return mech_##NS::cu_l[i]; }           \
-    static __device__ int cu_u(int i) { return mech_##NS::cu_u[i]; }           \
-    static __device__ int sf_sp(int i) { return mech_##NS::sf_sp[i]; }         \
-    static __device__ int sf_l(int i) { return mech_##NS::sf_l[i]; }           \
-    static __device__ int sf_x(int i) { return mech_##NS::sf_x[i]; }           \
-    static __device__ int sf_batch(int i) { return mech_##NS::sf_batch[i]; }   \
-    static __device__ int sb_sp(int i) { return mech_##NS::sb_sp[i]; }         \
-    static __device__ int sb_l(int i) { return mech_##NS::sb_l[i]; }           \
-    static __device__ int sb_x(int i) { return mech_##NS::sb_x[i]; }           \
-    static __device__ int sb_batch(int i) { return mech_##NS::sb_batch[i]; }   \
-    static constexpr int NSF_BATCH = mech_##NS::NSF_BATCH;                     \
-    static constexpr int NSB_BATCH = mech_##NS::NSB_BATCH;                     \
-  };
-GCKPP_MECH_TRAITS(fullchem)
-GCKPP_MECH_TRAITS(Hg)
-GCKPP_MECH_TRAITS(carbon)
-
-// Concentration of species s (variable from y, fixed from the conc array).
-template <class M, class CY, class CF>
-__device__ inline double spc(const CY &y, const CF &fix, int s) {
-  return s < M::NVAR ? (double)y[s] : (double)fix[s];
-}
-
 // acc + sum_{t0<=t<t1} w(t)*v(t), added in increasing t.  With U > 1 the
 // operands of U terms are loaded before their FMAs, so a wave keeps up to 2U
 // loads in flight instead of waiting on each term; the sum (and its rounding)
@@ -488,15 +428,6 @@ __device__ void dev_solve(const CL &lu, CX x) {
 }
 
 // ---------------------------------------------------------------------------
-// Rodas3 (gckpp_Integrator.F90:2239-2303): stages s=1..4; A, C lower
-// triangular row-wise; M, E solution/error weights; gamma(1)=0.5.
-__constant__ double kRosA[6] = {0.0, 2.0, 0.0, 2.0, 0.0, 1.0};
-__constant__ double kRosC[6] = {4.0, 1.0, -1.0, 1.0, -1.0, -(8.0 / 3.0)};
-__constant__ double kRosM[4] = {2.0, 0.0, 1.0, 1.0};
-__constant__ double kRosE[4] = {0.0, 0.0, 0.0, 1.0};
-__constant__ int kRosNewF[4] = {1, 0, 1, 1};
-constexpr double kRosGamma1 = 0.5;
-constexpr double kRosELO = 3.0;
 
 template <class M>
 __global__ __launch_bounds__(64) void k_integrate(IntegrateArgs a) {
@@ -1126,17 +1057,20 @@ const void *persist_kernel_variant(int qset, int variant) {
     case 4: return persist_kernel<16, 4, false>(qset);
     case 5: return persist_kernel<16, 4, true>(qset);
     case 6: return persist_kernel<8, 5, false>(qset);
+    case PERSIST_WAVE_VARIANT: return wave_kernel(qset);
   }
   return nullptr;
 }
 int persist_variant_info(int variant, int *U, int *W) {
-  static const int tab[][2] = {{1, 1}, {8, 1}, {8, 4}, {8, 4}, {16, 4}, {16, 4}, {8, 5}};
+  static const int tab[][2] = {{1, 1}, {8, 1}, {8, 4}, {8, 4}, {16, 4}, {16, 4}, {8, 5}, {0, 0}};
   if (variant < 0 || variant >= PERSIST_VARIANTS) return -1;
   *U = tab[variant][0];
   *W = tab[variant][1];
   return 0;
 }
-int persist_variant_seg(int variant) { return variant == 3 || variant == 5; }
+int persist_variant_seg(int variant) {
+  return variant == PERSIST_WAVE_VARIANT ? 2 : (variant == 3 || variant == 5);
+}
 int64_t persist_slot_fields(int qset) {
   switch (qset) {
     case PQ_FULLCHEM: return SlotLayout<Traits_fullchem>::FIELDS;
@@ -1157,6 +1091,7 @@ int64_t persist_resident_waves(int qset, int variant, int wpb) {
   const void *k = persist_kernel_variant(qset, variant);
   int dev = 0, ncu = 0, per = 0;
   if (!k || hipGetDevice(&dev) != hipSuccess) return 0;
+  if (variant == PERSIST_WAVE_VARIANT && wpb != 1) return 0;  // its workgroup is one wave
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64 * wpb, 0) != hipSuccess) return 0;
   return (int64_t)ncu * (per > 0 ? per : 1) * wpb;
@@ -1168,6 +1103,7 @@ int launch_persist(int qset, int variant, int wpb, const PersistArgs &pa, int64_
                    hipStream_t s) {
   const void *k = persist_kernel_variant(qset, variant);
   if (!k || nwaves <= 0 || wpb < 1 || wpb > 4 || nwaves % wpb) return -1;
+  if (variant == PERSIST_WAVE_VARIANT && wpb != 1) return -1;
   if (pa.counter) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.counter);
   void *args[] = {(void *)&pa};
   return hipLaunchKernel(k, dim3((unsigned)(nwaves / wpb)), dim3(64 * wpb), args, 0, s) == hipSuccess

@@ -1,0 +1,544 @@
+// Rodas3 solve with one wavefront per cell.  The lane-per-cell solvers
+// (rosenbrock.hip) keep each cell's state in HBM and stream ~1.3 MB per
+// Rodas3 attempt; here the cell's LU, solve vector, state and reaction
+// scratch live in LDS (fullchem: 71.6 KB, two cells per CU) and the 64 lanes
+// share its sums.  Every sum -- a Vdot row, a Jacobian entry, an LU position
+// (row-Crout form), a substitution row -- is computed by one lane, term by
+// term in the same order and with the same operations as the lane-per-cell
+// solvers, so the results are bit-identical to theirs.  The mechanism's wave
+// programs (gen/mech_<name>.h, built by mechgen/wave_sched.py) say which lane
+// computes which chunk of which sum at which dependency level.
+//
+// Algorithm: that of lane_attempt in rosenbrock.hip (ros_Integrator,
+// KPP/fullchem/gckpp_Integrator.F90:578-786, Rodas3 :2239-2303).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+
+#include "gckpp_internal.h"
+#include "mech_traits.h"
+
+namespace gckpp {
+namespace {
+
+constexpr unsigned kIdle = 0xFFFFu;
+
+// Orders the wave's LDS accesses across lanes: LDS instructions of one
+// wavefront execute in program order, so it suffices that the compiler moves
+// no access across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Global scratch written by some lanes and read by others (the rate
+// constants): wait for the wave's stores first.
+__device__ __forceinline__ void wave_sync_global() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Runs one wave program.  MODE 0 (Vdot, Jacobian): a chunk's sum starts at
+// +0.0 and a step is acc = fma(b, a, acc); MODE 1 (LU, substitution): it
+// starts at the destination slot and a step is acc = fma(-a, b, acc).
+// store(dest, hi, acc) writes a chunk's result.  Rounds go in one flat loop
+// over the program's levels; the head word and first block of 8 term words
+// of the next round are loaded while the current round runs (the arrays end
+// with an idle round and a block of pads, so these loads stay inside them).
+template <int MODE, int ZERO, int COEF0, class Store>
+__device__ __forceinline__ void run_prog(double *sh, const WProg &p, int lane, Store store) {
+  // pad step: operand a = ZERO (+0.0), b = ZERO or the -0.0 coefficient
+  constexpr unsigned PAD = (unsigned)ZERO | ((unsigned)(MODE == 0 ? COEF0 : ZERO) << 16);
+  const int R = p.lev[p.nlev];
+  int l = 0, lend = p.nlev > 0 ? p.lev[1] : 0;
+  unsigned h = p.head[lane];
+  unsigned w[8];
+  {
+    const unsigned *tp = p.term + (size_t)p.rs[0] * 64 + lane;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) w[u] = tp[u * 64];
+  }
+#pragma unroll 1
+  for (int r = 0; r < R; ++r) {
+    const int s0 = p.rs[r], n = p.rs[r + 1] - s0;
+    // next round's head and first block
+    const unsigned hn = p.head[(r + 1) * 64 + lane];
+    unsigned wn[8];
+    {
+      const unsigned *tq = p.term + (size_t)(s0 + n) * 64 + lane;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) wn[u] = tq[u * 64];
+    }
+    const unsigned dest = h & 0xFFFFu;
+    const bool act = dest != kIdle;
+    double acc = 0.0;
+    if constexpr (MODE == 1) acc = sh[act ? dest : (unsigned)ZERO];
+    const unsigned *tp = p.term + (size_t)s0 * 64 + lane;
+#pragma unroll 1
+    for (int s = 0; s < n; s += 8) {
+      if (s > 0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = tp[(s + u) * 64];
+      }
+      // past the round's last step the words belong to the next round or
+      // the final pads: use the pad (product -0.0)
+#pragma unroll
+      for (int u = 1; u < 8; ++u)
+        if (s + u >= n) w[u] = PAD;
+      double a[8], b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        a[u] = sh[w[u] & 0xFFFFu];
+        b[u] = sh[w[u] >> 16];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = MODE == 0 ? fma(b[u], a[u], acc) : fma(-a[u], b[u], acc);
+    }
+    if (act) store(dest, h >> 16, acc);
+    if (r + 1 == lend) {  // end of a level: its results become the next level's operands
+      wave_sync();
+      ++l;
+      if (l < p.nlev) lend = p.lev[l + 1];
+    }
+    h = hn;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) w[u] = wn[u];
+  }
+}
+
+// Per-cell integrator state, the same on every lane.
+struct WaveState {
+  double T, H, Hnew, Hexit, Texit, HnewAcc;
+  int st[8];
+  int ierr, tries;
+  bool rejLast, rejMore, newStep;
+};
+
+// The cell a wave works on: LDS block sh (layout WV_* of the mechanism) and
+// the wave's global scratch g: rate constants [NREACT], Fcn0 [NVAR], the
+// stage vectors K [4][NVAR].
+template <class M>
+struct WaveCell {
+  double *sh;
+  double *g;
+  int lane;
+  __device__ double *rct() const { return g; }
+  __device__ double *F0() const { return g + M::NREACT; }
+  __device__ double *K(int s) const { return g + M::NREACT + (1 + s) * M::NVAR; }
+};
+
+// Fun: A(r) = RCT(r)*prod conc(reactants) into the scratch, then Vdot into
+// X; variable species from the LDS vector at in0, fixed species from Y.
+template <class M>
+__device__ void wave_fun(const WaveCell<M> &c, int in0) {
+  double *sh = c.sh;
+  const double *rct = c.rct();
+#pragma unroll 1
+  for (int r = c.lane; r < M::NREACT; r += 64) {
+    double v = rct[r];
+#pragma unroll
+    for (int q = 0; q < M::RX_MAX; ++q) {
+      const int s = M::rx_s(r * M::RX_MAX + q);
+      if (s < 0) break;
+      v *= s < M::NVAR ? sh[in0 + s] : sh[M::WV_Y0 + s];
+    }
+    sh[M::WV_AB0 + r] = v;
+  }
+  wave_sync();
+  run_prog<0, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_FV), c.lane,
+                          [&](unsigned d, unsigned, double acc) { sh[d] = acc; });
+}
+
+// Ghimj = sign*JVS + diag on the diagonal into the LU block (partials from Y
+// into the scratch first); returns IER of KppDecomp's pivot check.
+template <class M>
+__device__ int wave_jac(const WaveCell<M> &c, double sign, double diag) {
+  double *sh = c.sh;
+  const double *rct = c.rct();
+#pragma unroll 1
+  for (int p = c.lane; p < M::NPART; p += 64) {
+    double v = M::pt_m(p) * rct[M::pt_r(p)];
+#pragma unroll
+    for (int q = 0; q < M::RX_MAX - 1; ++q) {
+      const int s = M::pt_s(p * (M::RX_MAX - 1) + q);
+      if (s < 0) break;
+      v *= sh[M::WV_Y0 + s];
+    }
+    sh[M::WV_AB0 + p] = v;
+  }
+  wave_sync();
+  run_prog<0, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_JV), c.lane, [&](unsigned d, unsigned isdiag, double acc) {
+    double v = sign * acc;
+    if (isdiag) v = v + diag;
+    sh[d] = v;
+  });
+  // the first row (1-based) whose diagonal was below TINY before its
+  // elimination
+#pragma unroll 1
+  for (int k0 = 0; k0 < M::NVAR; k0 += 64) {
+    const int k = k0 + c.lane;
+    const bool bad = k < M::NVAR && fabs(sh[M::lu_diag(k < M::NVAR ? k : 0)]) < DBL_MIN;
+    const unsigned long long mask = __ballot(bad);
+    if (mask) return k0 + __builtin_ctzll(mask) + 1;
+  }
+  return 0;
+}
+
+template <class M>
+__device__ void wave_decomp(const WaveCell<M> &c) {
+  double *sh = c.sh;
+  run_prog<1, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_LU), c.lane, [&](unsigned d, unsigned dv, double acc) {
+    if (dv) acc = acc / sh[dv - 1];
+    sh[d] = acc;
+  });
+}
+
+template <class M>
+__device__ void wave_solve(const WaveCell<M> &c) {
+  double *sh = c.sh;
+  run_prog<1, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_SF), c.lane,
+                          [&](unsigned d, unsigned, double acc) { sh[d] = acc; });
+  run_prog<1, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_SB), c.lane, [&](unsigned d, unsigned dv, double acc) {
+    if (dv) acc = acc / sh[dv - 1];
+    sh[d] = acc;
+  });
+}
+
+template <class M>
+__device__ void wave_begin(const IntegrateArgs &a, const WaveCell<M> &c, int64_t cell, bool cold,
+                           WaveState &S) {
+  double *sh = c.sh;
+  double *rct = c.rct();
+#pragma unroll 1
+  for (int s = c.lane; s < M::NSPEC; s += 64) sh[M::WV_Y0 + s] = a.conc[(int64_t)s * a.cstride + cell];
+#pragma unroll 1
+  for (int r = c.lane; r < M::NREACT; r += 64) rct[r] = a.rconst[(int64_t)r * a.rstride + cell];
+#pragma unroll 1
+  for (int i = c.lane; i < M::WV_NCOEF; i += 64) sh[M::WV_COEF0 + i] = M::wv_coef(i);
+  if (c.lane == 0) sh[M::WV_ZERO] = 0.0;
+  wave_sync_global();
+  const RosParams &P = a.prm;
+  const double hs = (a.hstart && !cold) ? a.hstart[cell] : 0.0;
+  double Hstart = P.Hstart_default;
+  if (hs > 0.0) Hstart = min_f(fabs(hs), fabs(P.Tend - P.Tstart));
+  S.T = P.Tstart;
+  double H = min_f(max_f(fabs(P.Hmin), fabs(Hstart)), fabs(P.Hmax));
+  if (fabs(H) <= 10.0 * P.roundoff) H = P.DeltaMin;
+  S.H = P.direction * H;
+  S.Hnew = S.H;
+  S.Hexit = S.Texit = S.HnewAcc = 0.0;
+  for (int k = 0; k < 8; ++k) S.st[k] = 0;
+  S.ierr = 1;
+  S.rejLast = S.rejMore = false;
+  S.newStep = true;
+}
+
+// One Rodas3 attempt of the wave's cell (lane_attempt's algorithm); true
+// when the cell is done (S.ierr then holds IERR).
+template <class M>
+__device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64_t cell,
+                             WaveState &S) {
+  constexpr int N = M::NVAR;
+  double *sh = c.sh;
+  const int lane = c.lane;
+  const RosParams &P = a.prm;
+  const double Tend = P.Tend, dir = P.direction;
+  double *F0 = c.F0();
+
+  if (S.newStep) {
+    const bool more = (dir > 0) ? ((S.T - Tend) + P.roundoff <= 0.0) : ((Tend - S.T) + P.roundoff <= 0.0);
+    if (!more) return true;
+    if (S.st[2] > P.max_steps) { S.ierr = -6; return true; }
+    if ((S.T + 0.1 * S.H) == S.T || S.H <= P.roundoff) { S.ierr = -7; return true; }
+    S.H = min_f(S.H, fabs(Tend - S.T));
+    wave_fun<M>(c, M::WV_Y0);
+#pragma unroll 1
+    for (int i = lane; i < N; i += 64) F0[i] = sh[M::WV_X0 + i];
+    S.st[0] += P.autonomous ? 1 : 2;
+    S.st[1] += 1;
+  }
+  int nconsec = 0;
+  bool singular = true;
+  while (singular) {
+    const int ising = wave_jac<M>(c, -1.0, 1.0 / (dir * S.H * kRosGamma1));
+    if (ising == 0) wave_decomp<M>(c);
+    S.st[5] += 1;
+    if (ising == 0) {
+      singular = false;
+    } else {
+      S.st[7] += 1;
+      ++nconsec;
+      if (nconsec <= 5) S.H *= 0.5;
+      else break;
+    }
+  }
+  if (singular) { S.ierr = -8; return true; }
+  const double H = S.H;
+#pragma unroll 1
+  for (int s = 0; s < 4; ++s) {
+    if (s == 0 || !kRosNewF[s]) {
+#pragma unroll 1
+      for (int i = lane; i < N; i += 64) sh[M::WV_X0 + i] = F0[i];
+    } else {
+#pragma unroll 1
+      for (int i = lane; i < N; i += 64) {
+        double v = sh[M::WV_Y0 + i];
+        for (int j = 0; j < s; ++j) {
+          const double aj = kRosA[s * (s - 1) / 2 + j];
+          if (aj != 0.0) v = fma(aj, c.K(j)[i], v);
+        }
+        sh[M::WV_YN0 + i] = v;
+      }
+      wave_sync();
+      wave_fun<M>(c, M::WV_YN0);
+      S.st[0] += 1;
+    }
+    for (int j = 0; j < s; ++j) {
+      const double hc = kRosC[s * (s - 1) / 2 + j] / (dir * H);
+      if (hc != 0.0) {
+        const double *Kj = c.K(j);
+#pragma unroll 1
+        for (int i = lane; i < N; i += 64) sh[M::WV_X0 + i] = fma(hc, Kj[i], sh[M::WV_X0 + i]);
+      }
+    }
+    wave_sync();
+    wave_solve<M>(c);
+    double *Ks = c.K(s);
+#pragma unroll 1
+    for (int i = lane; i < N; i += 64) Ks[i] = sh[M::WV_X0 + i];
+    S.st[6] += 1;
+  }
+  // Ynew and the scaled error terms per species, then the error sum in
+  // species order (every lane sums the same terms)
+#pragma unroll 1
+  for (int i = lane; i < N; i += 64) {
+    const double y0 = sh[M::WV_Y0 + i];
+    double v = y0;
+    for (int j = 0; j < 4; ++j)
+      if (kRosM[j] != 0.0) v = fma(kRosM[j], c.K(j)[i], v);
+    double ye = 0.0;
+    for (int j = 0; j < 4; ++j)
+      if (kRosE[j] != 0.0) ye = fma(kRosE[j], c.K(j)[i], ye);
+    const double ymax = max_f(fabs(y0), fabs(v));
+    const double sc = P.vectol ? a.atol[i] + a.rtol[i] * ymax : a.atol[0] + a.rtol[0] * ymax;
+    sh[M::WV_YN0 + i] = v;
+    sh[M::WV_AB0 + i] = ye / sc;
+  }
+  wave_sync();
+  double err = 0.0;
+  {
+    int i = 0;
+#pragma unroll 1
+    for (; i + 8 <= N; i += 8) {
+      double q[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) q[u] = sh[M::WV_AB0 + i + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) err = fma(q[u], q[u], err);
+    }
+#pragma unroll 1
+    for (; i < N; ++i) {
+      const double q = sh[M::WV_AB0 + i];
+      err = fma(q, q, err);
+    }
+  }
+  err = max_f(sqrt(err / N), 1.0e-10);
+  const double fac = min_f(P.FacMax, max_f(P.FacMin, P.FacSafe / pow(err, 1.0 / kRosELO)));
+  S.Hnew = H * fac;
+  if (a.hlog && S.st[2] < a.hlog_n && lane == 0) {
+    a.hlog[(int64_t)(2 * S.st[2]) * a.ncell + cell] = (float)H;
+    a.hlog[(int64_t)(2 * S.st[2] + 1) * a.ncell + cell] = (float)err;
+  }
+  S.st[2] += 1;
+  if (err <= 1.0 || H <= P.Hmin) {
+    S.st[3] += 1;
+#pragma unroll 1
+    for (int i = lane; i < N; i += 64) {
+      const double v = sh[M::WV_YN0 + i];
+      sh[M::WV_Y0 + i] = P.clip_negative ? max_f(v, 0.0) : v;
+    }
+    S.T = S.T + dir * H;
+    S.Hnew = max_f(P.Hmin, min_f(S.Hnew, P.Hmax));
+    if (S.rejLast) S.Hnew = min_f(S.Hnew, H);
+    S.Hexit = H;
+    S.Texit = S.T;
+    S.HnewAcc = S.Hnew;
+    S.rejLast = S.rejMore = false;
+    S.H = S.Hnew;
+    S.newStep = true;
+  } else {
+    if (S.rejMore) S.Hnew = H * P.FacRej;
+    S.rejMore = S.rejLast;
+    S.rejLast = true;
+    S.H = S.Hnew;
+    if (S.st[3] >= 1) S.st[4] += 1;
+    S.newStep = false;
+  }
+  wave_sync();
+  return false;
+}
+
+// The carbon mechanism's forward-Euler integration (lane_euler's algorithm).
+template <class M>
+__device__ void wave_euler(const IntegrateArgs &a, const WaveCell<M> &c, WaveState &S) {
+  double *sh = c.sh;
+  wave_fun<M>(c, M::WV_Y0);
+  const double dt = a.prm.Tend - a.prm.Tstart;
+  bool neg = false;
+#pragma unroll 1
+  for (int i = c.lane; i < M::NVAR; i += 64) neg = neg || (sh[M::WV_Y0 + i] + sh[M::WV_X0 + i] * dt < 0.0);
+  neg = __ballot(neg) != 0;
+  if ((a.prm.neg_mode == 2 || a.prm.neg_mode == 3) && neg) {
+    S.ierr = -9;
+    return;
+  }
+#pragma unroll 1
+  for (int i = c.lane; i < M::NVAR; i += 64) {
+    double v = sh[M::WV_Y0 + i] + sh[M::WV_X0 + i] * dt;
+    if (a.prm.neg_mode == 1 && v < 0.0) v = 0.0;
+    sh[M::WV_Y0 + i] = v;
+  }
+  wave_sync();
+  S.ierr = 1;
+}
+
+template <class M>
+__device__ void wave_finish(const IntegrateArgs &a, const WaveCell<M> &c, int64_t cell,
+                            const WaveState &S) {
+  const double *sh = c.sh;
+  const int nout = a.clip_out ? M::NSPEC : M::NVAR;
+#pragma unroll 1
+  for (int s = c.lane; s < nout; s += 64) {
+    double v = sh[M::WV_Y0 + s];
+    if (a.clip_out) v = v > 0.0 ? v : 0.0;
+    a.conc[(int64_t)s * a.cstride + cell] = v;
+  }
+  const int k = c.lane;
+  int stk = 0;  // S.st[k] without a lane-indexed (scratch) array access
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (k == j) stk = S.st[j];
+  if (a.istatus && k < 20) a.istatus[(int64_t)k * a.sstride + cell] = stk;
+  if (a.rstatus && k < 20) {
+    const double v = k == 0 ? S.Texit : k == 1 ? S.Hexit : k == 2 ? S.HnewAcc : 0.0;
+    a.rstatus[(int64_t)k * a.sstride + cell] = v;
+  }
+  if (k == 0) {
+    if (a.hnew) a.hnew[cell] = S.HnewAcc;
+    if (a.ierr) a.ierr[cell] = S.ierr;
+  }
+}
+
+template <class M0, class M1, class M2, class Fn>
+__device__ __forceinline__ void on_queue_w(int q, Fn &&f) {
+  if constexpr (!std::is_void_v<M2>) {
+    if (q == 2) { f((M2 *)nullptr); return; }
+  }
+  if constexpr (!std::is_void_v<M1>) {
+    if (q == 1) { f((M1 *)nullptr); return; }
+  }
+  f((M0 *)nullptr);
+}
+
+template <class M> struct WvSh { static constexpr int v = M::WV_SH; };
+template <> struct WvSh<void> { static constexpr int v = 0; };
+template <class M> struct IsEulerW : std::false_type {};
+template <> struct IsEulerW<Traits_carbon> : std::true_type {};
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// One wave (= one workgroup) per resident slot; cells from the queues of
+// PersistArgs by work stealing (or one cell per wave when counter is null),
+// with the step-mode retry, clip and RSTATUS(3) write-back of k_persist.
+template <class M0, class M1, class M2>
+__global__ __launch_bounds__(64) void k_wave(PersistArgs pa) {
+  constexpr int SH = cmax(WvSh<M0>::v, cmax(WvSh<M1>::v, WvSh<M2>::v));
+  __shared__ double sh[SH];
+  const int lane = threadIdx.x;
+  const int64_t wave = blockIdx.x;
+  double *g = pa.work + wave * (64 * pa.slot_fields);
+  constexpr int NQ = std::is_void_v<M1> ? 1 : (std::is_void_v<M2> ? 2 : 3);
+  const int64_t total = pa.qend[NQ - 1];
+  int64_t cell = -1;
+  int q = 0;
+  int64_t next_static = wave;
+  WaveState S;
+  for (;;) {
+    if (cell < 0) {
+      int64_t cidx;
+      if (pa.counter) {
+        unsigned v = 0;
+        if (lane == 0) v = atomicAdd(pa.counter, 1u);
+        cidx = (int64_t)__builtin_amdgcn_readfirstlane(v);
+      } else {
+        cidx = next_static;
+        next_static += gridDim.x;
+      }
+      if (cidx >= total) break;
+      // queue of the cell: the number of queue ends at or below cidx (the
+      // ends are nondecreasing).  Written as a sum of compares: the
+      // equivalent early-exit loop was compiled for this wave-uniform cidx
+      // into a select chain that gave the first queue for cells of the
+      // second.
+      q = 0;
+#pragma unroll
+      for (int k = 0; k < NQ - 1; ++k) q += cidx >= pa.qend[k] ? 1 : 0;
+      cell = cidx - (q ? pa.qend[q - 1] : 0);
+      if (pa.q[q].order) cell = pa.q[q].order[cell];
+      S.tries = 0;
+      on_queue_w<M0, M1, M2>(q, [&](auto *tag) {
+        using M = std::remove_pointer_t<decltype(tag)>;
+        wave_begin<M>(pa.q[q], WaveCell<M>{sh, g, lane}, cell, false, S);
+      });
+    }
+    bool done = false;
+    on_queue_w<M0, M1, M2>(q, [&](auto *tag) {
+      using M = std::remove_pointer_t<decltype(tag)>;
+      const WaveCell<M> c{sh, g, lane};
+      if constexpr (IsEulerW<M>::value) {
+        wave_euler<M>(pa.q[q], c, S);
+        done = true;
+      } else {
+        done = wave_attempt<M>(pa.q[q], c, cell, S);
+      }
+    });
+    if (done) {
+      const IntegrateArgs &a = pa.q[q];
+      if (S.ierr < 0 && a.retry && S.tries == 0) {
+        on_queue_w<M0, M1, M2>(q, [&](auto *tag) {
+          using M = std::remove_pointer_t<decltype(tag)>;
+          wave_begin<M>(a, WaveCell<M>{sh, g, lane}, cell, true, S);
+        });
+        S.tries = 1;
+      } else {
+        on_queue_w<M0, M1, M2>(q, [&](auto *tag) {
+          using M = std::remove_pointer_t<decltype(tag)>;
+          wave_finish<M>(a, WaveCell<M>{sh, g, lane}, cell, S);
+        });
+        wave_sync();
+        cell = -1;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// The cell-per-wave kernel for a queue set (PQ_*), launched with 64 threads
+// per workgroup.
+const void *wave_kernel(int qset) {
+  switch (qset) {
+    case PQ_FULLCHEM: return (const void *)k_wave<Traits_fullchem, void, void>;
+    case PQ_HG: return (const void *)k_wave<Traits_Hg, void, void>;
+    case PQ_CARBON: return (const void *)k_wave<Traits_carbon, void, void>;
+    case PQ_MIXED: return (const void *)k_wave<Traits_fullchem, Traits_Hg, Traits_carbon>;
+  }
+  return nullptr;
+}
+
+}  // namespace gckpp

@@ -32,7 +32,8 @@ SYMBOLS = [
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
-PERSIST_VARIANTS = 7        # gckpp_persistent_variants(); checked by the CPU tests
+PERSIST_VARIANTS = 8        # gckpp_persistent_variants(); checked by the CPU tests
+WAVE_VARIANT = 7            # one wavefront per cell, state in LDS (64-thread workgroups)
 
 
 class Part(ctypes.Structure):
@@ -186,11 +187,15 @@ class Context:
         _check(self.L.gckpp_set_step_log(self.h, _ptr(hlog), int(nattempt)), "gckpp_set_step_log")
 
     def persistent_config(self):
-        """dict(variant, waves_per_group, waves, loads_in_flight, min_waves_per_simd)."""
+        """dict(variant, waves_per_group, waves, loads_in_flight,
+        min_waves_per_simd, segments, wave_per_cell)."""
         out = (ctypes.c_int32 * 6)()
         _check(self.L.gckpp_get_persistent_config(self.h, out), "gckpp_get_persistent_config")
-        return dict(zip(("variant", "waves_per_group", "waves", "loads_in_flight",
-                         "min_waves_per_simd", "segments"), list(out)))
+        d = dict(zip(("variant", "waves_per_group", "waves", "loads_in_flight",
+                      "min_waves_per_simd"), list(out)[:5]))
+        d["segments"] = int(out[5] == 1)
+        d["wave_per_cell"] = int(out[5] == 2)
+        return d
 
     def autotune(self, conc, rconst, hstart, tin, tout, icntrl, rcntrl):
         """Times every persistent configuration on these cells (conc is not

@@ -18,11 +18,16 @@ wavefront-uniform index, so the compiler reads it through the scalar cache.
                         lp in [le_p0, le_p1)
   cu_*, sf_*, sb_*      the same sums as flat segment lists (see build_tables)
                         for the persistent solver's batched execution
+  wv_*, WV_*            the cell-per-wave solver's programs and LDS layout
+                        (wave_sched.py)
 Run:  python emit_tables.py <mech> [<mech> ...]
 """
 import json
 import sys
 from pathlib import Path
+
+sys.path.insert(0, str(Path(__file__).resolve().parent))
+import wave_sched  # noqa: E402
 
 HERE = Path(__file__).resolve().parent
 RX_MAX = 3
@@ -35,7 +40,8 @@ def arr(ctype, name, vals, per_line=16, fmt=str):
     lines = []
     for i in range(0, len(vals), per_line):
         lines.append("  " + ", ".join(fmt(v) for v in vals[i:i + per_line]) + ",")
-    return "__constant__ %s %s[%d] = {\n%s\n};\n" % (ctype, name, len(vals), "\n".join(lines))
+    # internal linkage: the header is included by more than one translation unit
+    return "static __constant__ %s %s[%d] = {\n%s\n};\n" % (ctype, name, len(vals), "\n".join(lines))
 
 
 def dfmt(v):
@@ -216,6 +222,24 @@ def emit(name):
         out.append(arr("double", k, t[k], per_line=6, fmt=dfmt))
     for k, key in (("lu_crow", "lu_crow"), ("lu_diag", "lu_diag"), ("lu_icol", "lu_icol")):
         out.append(arr("int", k, m[key][: (m["nvar"] + 1) if k != "lu_icol" else None]))
+    # cell-per-wave solver: LDS layout and programs
+    prog = wave_sched.build_wave_program(m, t)
+    L = prog["layout"]
+    out.append("constexpr int WV_ZERO = %d, WV_X0 = %d, WV_Y0 = %d, WV_YN0 = %d, WV_AB0 = %d;"
+               % (L["ZERO"], L["X0"], L["Y0"], L["YN0"], L["AB0"]))
+    out.append("constexpr int WV_COEF0 = %d, WV_NCOEF = %d, WV_SH = %d;"
+               % (L["COEF0"], len(prog["coef"]), L["SH_SIZE"]))
+    nlev = {P: (len(prog[P][0][0]) - 1 if P in prog else 0) for P in ("fv", "jv", "lu", "sf", "sb")}
+    out.append("constexpr int WV_FV_NLEV = %d, WV_JV_NLEV = %d, WV_LU_NLEV = %d, WV_SF_NLEV = %d, "
+               "WV_SB_NLEV = %d;" % tuple(nlev[P] for P in ("fv", "jv", "lu", "sf", "sb")))
+    have = {P for P in ("fv", "jv", "lu", "sf", "sb") if P in prog}
+    for aname, ctype, vals in wave_sched.emit_arrays(prog):
+        out.append(arr(ctype, aname, vals, per_line=6 if ctype == "double" else 16,
+                       fmt=dfmt if ctype == "double" else str))
+    for P in ("fv", "jv", "lu", "sf", "sb"):   # empty programs of a mechanism without an LU
+        if P not in have:
+            for suf, ctype in (("lev", "int"), ("rs", "int"), ("head", "unsigned"), ("term", "unsigned")):
+                out.append(arr(ctype, "wv_%s_%s" % (P, suf), [0]))
     out.append("}  // namespace mech_%s" % name)
     gen = HERE.parent / "csrc" / "gen"
     gen.mkdir(exist_ok=True)

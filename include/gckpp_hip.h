@@ -125,14 +125,16 @@ int gckpp_step_timings(gckpp_ctx *ctx, float ms[4]);
 
 /* Persistent solver configuration (no reference counterpart; the OpenMP
  * schedule of the cell loop, GeosCore/fullchem_mod.F90:528-546, is the CPU
- * analogue).  variant in [0, gckpp_persistent_variants()): (U, W, S) =
- * terms per batched gather U, minimum waves per SIMD W (the register
- * budget), S = 1: the independent sums (Fun, Jac, the U part of the LU,
- * substitution in batches of independent rows) as flat segment lists:
- * (1,1,0) (8,1,0) (8,4,0) (8,4,1) (16,4,0) (16,4,1) (8,5,0).  All variants
- * add the same terms in the same order.  waves_per_group 1, 2 or 4 (cells
- * per workgroup 64, 128, 256).  get: out = {variant, waves_per_group, waves
- * per launch, U, W, S}. */
+ * analogue).  variant in [0, gckpp_persistent_variants()).  Variants 0-6:
+ * one lane per cell, state in HBM; (U, W, S) = terms per batched gather U,
+ * minimum waves per SIMD W (the register budget), S = 1: the independent
+ * sums (Fun, Jac, the U part of the LU, substitution in batches of
+ * independent rows) as flat segment lists: (1,1,0) (8,1,0) (8,4,0) (8,4,1)
+ * (16,4,0) (16,4,1) (8,5,0); waves_per_group 1, 2 or 4 (cells per
+ * workgroup 64, 128, 256).  Variant 7: one wavefront per cell with the
+ * cell's state in LDS (waves_per_group must be 1; U = W = 0, S = 2).  All
+ * variants add the same terms in the same order.  get: out = {variant,
+ * waves_per_group, waves per launch, U, W, S}. */
 int gckpp_persistent_variants(void);
 int gckpp_set_persistent_config(gckpp_ctx *ctx, int variant, int waves_per_group);
 int gckpp_get_persistent_config(gckpp_ctx *ctx, int32_t out[6]);

@@ -23,3 +23,15 @@ def fullchem():
 def hg():
     from gckpp_amd import mechanism
     return mechanism.load("Hg")
+
+
+@pytest.fixture(scope="session")
+def mech_of():
+    from gckpp_amd import mechanism
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            cache[name] = mechanism.load(name)
+        return cache[name]
+    return get
