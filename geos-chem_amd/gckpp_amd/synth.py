@@ -261,6 +261,24 @@ def perturb(mech, conc, met, dt=1200.0, seed=0, mixing=0.05):
     return out
 
 
+def perturb_dev(mech, conc, met, gen, dt=1200.0, mixing=0.05):
+    """perturb() on device tensors, in place: conc [NSPEC][n] and met
+    [NMET][n] torch tensors on the GPU, gen a torch.Generator on the same
+    device (the same distributions as perturb, other random draws)."""
+    import torch
+    nv = mech.nvar
+    n = conc.shape[1]
+    conc[:nv] *= torch.exp(torch.randn((nv, n), generator=gen, device=conc.device,
+                                       dtype=conc.dtype) * mixing)
+    bl = met[fields.MET["PRESS"][0]] > 700.0
+    for name, (lo, hi) in _EMIS.items():
+        if name in mech.species:
+            u = torch.rand(n, generator=gen, device=conc.device, dtype=conc.dtype)
+            e = 10.0 ** (np.log10(lo) + u * (np.log10(hi) - np.log10(lo)))
+            conc[mech.species.index(name)] += torch.where(bl, e * dt, torch.zeros_like(e))
+    return conc
+
+
 def mixed_grid(mechs, nx=144, ny=91, nz=72, seed=3, split=(0.7, 0.2, 0.1), hour_utc=12.0):
     """Cells of one nx*ny*nz grid assigned at random to the mechanisms mechs
     (e.g. fullchem, Hg, carbon) with probabilities split; returns one
