@@ -488,10 +488,28 @@ def mixed_bench(args):
     return out
 
 
+def _scaled_gpu(mech, c, ref):
+    """rel_errors' scaled-error fields of device tensors c vs ref [NSPEC][n]
+    (computed on the GPU: the C90 faces are too large to copy for this)."""
+    import torch
+    nv = mech.nvar
+    a = torch.from_numpy(mech.atol()).to(c.device)[:, None]
+    r = torch.from_numpy(mech.rtol()).to(c.device)[:, None]
+    sc = (c[:nv] - ref[:nv]).abs() / (a + r * ref[:nv].abs())
+    sig = ref[:nv].abs() > SIGNIFICANT
+    return {"max_scaled_err": float(sc.max().item()), "frac_within_rtol": float((sc <= 1.0).double().mean().item()),
+            "max_scaled_err_significant": float(sc[sig].max().item()) if bool(sig.any()) else 0.0,
+            "frac_within_rtol_significant": float((sc[sig] <= 1.0).double().mean().item())}
+
+
 def c90_bench(args):
-    """6x90x90x72 = 3,499,200 fullchem cells, one chemistry step, solved in
-    six face-sized chunks (583,200 cells) through one context whose
-    workspace fits HBM; learned h0 + persistent solver."""
+    """6x90x90x72 = 3,499,200 fullchem cells solved in six face-sized chunks
+    (583,200 cells) through one context whose workspace fits HBM.  Two
+    steps per leg: the cold step (Hstart = 1e-5 s), then the warm step that
+    follows it after a synthetic emission/transport perturbation of the
+    state (Hstart = KPPHvalue, or the learned h0 from the state and
+    KPPHvalue).  Legs: plain GPU Rosenbrock and learned h0 + persistent.
+    value = the learned leg's cells/s over the cold step."""
     import numpy as np
     import torch
     from gckpp_amd import hip, learned, mechanism, synth
@@ -500,56 +518,128 @@ def c90_bench(args):
     ic, rc = fullchem_icntrl()
     nface = 90 * 90 * 72
     ctx = hip.Context(mech, nface)
-    ctx.set_solver(hip.SOLVER_PERSISTENT)
     mpath = learned.default_model_path("fullchem")
     model = learned.StepModel(mech).load(mpath) if mpath.exists() else None
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    faces = []
+    faces, samples = [], []
+    pick = np.linspace(0, nface - 1, 1366).astype(np.int64)   # 8,196 sampled cells over the six faces
     for f in range(6):
         w = synth.grid_cells(mech, 90, 90, 72, seed=200 + f, hour_utc=(12.0 + 4.0 * f) % 24.0)
         faces.append(dict(met=dev(w.met), photol=dev(w.photol), het=dev(w.het), conc0=dev(w.conc)))
+        samples.append(w.subset(pick))
         log("face %d generated" % f)
+    del w
     h0 = torch.empty(nface, dtype=torch.float64, device="cuda")
     cls = torch.empty(nface, dtype=torch.int32, device="cuda")
     cold = torch.zeros(nface, dtype=torch.float64, device="cuda")
     conc = torch.empty_like(faces[0]["conc0"])
+    gen = torch.Generator(device="cuda")
 
-    def run(learned):
-        fails, att = 0, []
-        ctx.set_solver(hip.SOLVER_PERSISTENT if learned else hip.SOLVER_PLAIN)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for f in faces:
-            conc.copy_(f["conc0"])
-            hs = cold
+    def run(use_learned, warm):
+        """One step of every face; returns wall s, kernel ms, attempts, ..."""
+        fails, att, acc_att, kern_ms = 0, [], 0, 0.0
+        ctx.set_solver(hip.SOLVER_PERSISTENT if use_learned else hip.SOLVER_PLAIN)
+        outs = []
+        wall = 0.0
+        for k, f in enumerate(faces):
+            src = f["warm_in"] if warm else f["conc0"]
+            conc.copy_(src)
+            hprev = f["kpph"] if warm else None
+            hs = hprev if warm else cold
             ctx.set_cell_order(None)
-            if learned and model is not None:
-                model.predict(f["met"], f["photol"], conc, safety=args.safety, h0=h0, cls=cls)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if use_learned and model is not None:
+                model.predict(f["met"], f["photol"], conc, hprev=hprev, safety=args.safety, h0=h0, cls=cls)
                 ctx.set_cell_order(hip.order_by_class(cls))
                 hs = h0
             hnew, ist, ierr, ok = ctx.fullchem_step(f["met"], f["photol"], f["het"], conc, hs, 1200.0,
                                                     ic, rc)
+            torch.cuda.synchronize()
+            wall += time.perf_counter() - t0
+            kern_ms += float(ctx.step_timings()[1])
             fails += int((ierr < 0).sum().item())
             att.append(float(ist[2].float().mean().item()))
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+            acc_att += int(ist[3].sum().item())
+            tot_att = int(ist[2].sum().item())
+            outs.append((conc.clone(), hnew.clone(), acc_att, tot_att))
+            acc_att = 0
         ctx.set_cell_order(None)
-        log("c90 %s: %.1f s" % ("learned + persistent" if learned else "plain", wall))
-        return wall, fails, att
+        log("c90 %s %s step: %.1f s" % ("learned + persistent" if use_learned else "plain",
+                                         "warm" if warm else "cold", wall))
+        return wall, kern_ms, fails, att, outs
 
-    wall, fails, att = run(True)
-    wall_p, fails_p, att_p = run(False)
+    res = {}
+    for warm in (False, True):
+        if warm:
+            # the warm step's input: the plain leg's cold result, perturbed
+            for f, (c, hn, _, _) in zip(faces, res[("plain", False)][4]):
+                f["warm_in"] = c.clone()
+                synth.perturb_dev(mech, f["warm_in"], f["met"], gen.manual_seed(7))
+                f["kpph"] = hn
+        for leg in ("learned", "plain"):
+            res[(leg, warm)] = run(leg == "learned", warm)
     n = 6 * nface
-    return {"metric": METRIC + " (C90-sized)", "value": round(n / wall, 2), "unit": "cells/s", "n_gpus": 1,
-            "steps": 1, "warmup": 0, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True,
+
+    def leg_json(key):
+        wall, kms, fails, att, outs = res[key]
+        return {"value": round(n / wall, 2), "unit": "cells/s", "ms_per_step": round(wall * 1e3, 3),
+                "integrate_kernel_ms": round(kms, 3), "failed_cells": fails,
+                "mean_attempts_per_face": [round(a, 3) for a in att]}
+
+    # accuracy: learned vs plain on every cell (GPU), and both against the
+    # CPU reference and the RTOL 1e-8 oracle on samples of the cold step
+    acc = {}
+    for warm in (False, True):
+        errs = [_scaled_gpu(mech, a[0], b[0]) for a, b in zip(res[("learned", warm)][4], res[("plain", warm)][4])]
+        acc["learned_vs_plain_gpu_" + ("warm" if warm else "cold")] = {
+            "max_scaled_err": max(e["max_scaled_err"] for e in errs),
+            "frac_within_rtol": float(np.mean([e["frac_within_rtol"] for e in errs])),
+            "max_scaled_err_significant": max(e["max_scaled_err_significant"] for e in errs),
+            "frac_within_rtol_significant": float(np.mean([e["frac_within_rtol_significant"] for e in errs]))}
+    ws = samples[0]
+    for k in range(1, 6):
+        o = samples[k]
+        ws = synth.Workload(mech, np.concatenate([ws.met, o.met], 1), np.concatenate([ws.photol, o.photol], 1),
+                            np.concatenate([ws.het, o.het], 1) if ws.het.shape[1] > 1 else ws.het,
+                            np.concatenate([ws.conc, o.conc], 1), np.concatenate([ws.hstart, o.hstart]), ws.dt,
+                            "c90 faces")
+    gl = np.concatenate([res[("learned", False)][4][k][0][:, pick].cpu().numpy() for k in range(6)], 1)
+    gp = np.concatenate([res[("plain", False)][4][k][0][:, pick].cpu().numpy() for k in range(6)], 1)
+    base, sidx, refout = cpu_baseline(mech, ws, ic, rc, ws.ncell)
+    if refout is not None:
+        acc["learned_vs_reference_cold"] = rel_errors(mech, gl[:, sidx], refout[0])
+        acc["plain_vs_reference_cold"] = rel_errors(mech, gp[:, sidx], refout[0])
+        if args.tight_sample > 0:
+            tp = np.linspace(0, len(sidx) - 1, args.tight_sample).astype(np.int64)
+            cells = sidx[tp]
+            acc["vs_rtol_1e-8_oracle_cold"] = tight_check(
+                mech, ws, cells, {"learned": gl[:, cells], "plain_gpu": gp[:, cells],
+                                  "reference": refout[0][:, tp]}, ic, rc)
+    # roofline of the learned leg's integrate kernels over the cold step
+    flop = 0.0
+    for (_, _, nacc, ntot) in res[("learned", False)][4]:
+        flop += FLOP_ACC * nacc + FLOP_REJ * (ntot - nacc)
+    kms = res[("learned", False)][1]
+    achieved = flop / (kms * 1e-3) / 1e12
+    L, P = leg_json(("learned", False)), leg_json(("plain", False))
+    Lw, Pw = leg_json(("learned", True)), leg_json(("plain", True))
+    return {"metric": METRIC + " (C90-sized)", "value": L["value"], "unit": "cells/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": L["ms_per_step"], "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "6x90x90x72 = 3,499,200 fullchem cells (six 90x90x72 synthetic "
-                                   "faces, seeds 200-205), one step, six 583,200-cell chunks, learned "
-                                   "h0 + persistent", "learned_model": model is not None},
-            "failed_cells": fails, "mean_attempts_per_face": att,
-            "plain_gpu": {"value": round(n / wall_p, 2), "unit": "cells/s", "ms_per_step": round(wall_p * 1e3, 3),
-                          "failed_cells": fails_p, "mean_attempts_per_face": att_p},
-            "speedup_vs_plain": round(wall_p / wall, 3)}
+                                   "faces, seeds 200-205), six 583,200-cell chunks; cold step, then "
+                                   "the warm step after a synthetic emission/transport perturbation; "
+                                   "learned h0 + persistent against plain GPU",
+                       "learned_model": model is not None,
+                       "persistent_config": ctx.persistent_config()},
+            "learned": L, "plain_gpu": P, "speedup_vs_plain": round(P["ms_per_step"] / L["ms_per_step"], 3),
+            "warm_step": {"learned": Lw, "plain_kpphvalue": Pw,
+                          "speedup_vs_plain": round(Pw["ms_per_step"] / Lw["ms_per_step"], 3)},
+            "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6), "traffic": None,
+                         "kernel": "k_persist over the six chunks of the cold step"},
+            "accuracy": acc, "cpu_baseline": base}
 
 
 def main():
