@@ -248,9 +248,9 @@ def grid_bench(args):
     traffic = None
     tfile = ROOT / "profiles" / "persist_traffic.json"
     if tfile.exists():
-        tr = json.loads(tfile.read_text())
-        if tr.get("variant") == pcfg["variant"] and tr.get("cells") == n:
-            traffic = tr.get("bytes_per_launch")
+        for tr in json.loads(tfile.read_text()).get("entries", []):
+            if tr.get("variant") == pcfg["variant"] and tr.get("cells") == n:
+                traffic = tr.get("bytes_per_launch")
 
     base, sidx, refout = cpu_baseline(mech, w, ic, rc, args.cpu_sample)
     accuracy = {"learned_vs_plain_gpu": rel_errors(mech, L["conc"], P["conc"])}
