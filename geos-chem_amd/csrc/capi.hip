@@ -653,7 +653,7 @@ extern "C" int gckpp_set_step_log(gckpp_ctx *c, float *hlog, int nattempt) {
   return 0;
 }
 
-extern "C" int gckpp_get_persistent_config(gckpp_ctx *c, int32_t out[6]) {
+extern "C" int gckpp_get_persistent_config(gckpp_ctx *c, int32_t out[7]) {
   if (!c || !out) return fail("gckpp_get_persistent_config: null argument");
   int U = 0, W = 0;
   persist_variant_info(c->pvariant, &U, &W);
@@ -663,6 +663,7 @@ extern "C" int gckpp_get_persistent_config(gckpp_ctx *c, int32_t out[6]) {
   out[3] = U;
   out[4] = W;
   out[5] = persist_variant_seg(c->pvariant);
+  out[6] = persist_variant_cache(c->pvariant);
   return 0;
 }
 

@@ -96,9 +96,10 @@ inline int64_t integrate_work_per_cell(const MechInfo &m) {
 void launch_integrate_mech(int mech, const IntegrateArgs &a, hipStream_t s);
 // persistent work-stealing integrator (k_persist): queue sets and variants
 enum { PQ_FULLCHEM = 0, PQ_HG = 1, PQ_CARBON = 2, PQ_MIXED = 3 };
-constexpr int PERSIST_VARIANTS = 8;
+constexpr int PERSIST_VARIANTS = 10;
 constexpr int PERSIST_WAVE_VARIANT = 7;  // one wave per cell (wave_solver.hip)
 int persist_variant_info(int variant, int *U, int *W);
+int persist_variant_cache(int variant);  // operand-cache slots per lane (0: none)
 int persist_variant_seg(int variant);  // 0 lane per cell, 1 with flat segments, 2 wave per cell
 const void *wave_kernel(int qset);     // the cell-per-wave kernel (64 threads per workgroup)
 int64_t persist_slot_fields(int qset);                   // doubles per lane slot

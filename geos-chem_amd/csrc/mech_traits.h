@@ -27,6 +27,9 @@ enum { WP_FV = 0, WP_JV = 1, WP_LU = 2, WP_SF = 3, WP_SB = 4 };
     static constexpr int NSPEC = mech_##NS::NSPEC, NREACT = mech_##NS::NREACT; \
     static constexpr int LU_NZ = mech_##NS::LU_NZ, NPART = mech_##NS::NPART;   \
     static constexpr int RX_MAX = mech_##NS::RX_MAX;                           \
+    /* entries of the Jacobian-term and Crout-term tables */                   \
+    static constexpr int NJV_TAB = sizeof(mech_##NS::jv_b) / sizeof(int);      \
+    static constexpr int NCR_TAB = sizeof(mech_##NS::cr_l) / sizeof(int);      \
     static __device__ int rx_s(int i) { return mech_##NS::rx_s[i]; }           \
     static __device__ int fv_ptr(int i) { return mech_##NS::fv_ptr[i]; }       \
     static __device__ int fv_r(int i) { return mech_##NS::fv_r[i]; }           \
@@ -64,6 +67,12 @@ enum { WP_FV = 0, WP_JV = 1, WP_LU = 2, WP_SF = 3, WP_SB = 4 };
     static __device__ int sb_l(int i) { return mech_##NS::sb_l[i]; }           \
     static __device__ int sb_x(int i) { return mech_##NS::sb_x[i]; }           \
     static __device__ int sb_batch(int i) { return mech_##NS::sb_batch[i]; }   \
+    static __device__ unsigned lc_lu(int s, int i) {                           \
+      return s == 32 ? mech_##NS::lc32_lu[i] : mech_##NS::lc16_lu[i];          \
+    }                                                                          \
+    static __device__ unsigned lc_sv(int s, int i) {                           \
+      return s == 32 ? mech_##NS::lc32_sv[i] : mech_##NS::lc16_sv[i];          \
+    }                                                                          \
     static constexpr int NSF_BATCH = mech_##NS::NSF_BATCH;                     \
     static constexpr int NSB_BATCH = mech_##NS::NSB_BATCH;                     \
     static constexpr int WV_ZERO = mech_##NS::WV_ZERO, WV_X0 = mech_##NS::WV_X0; \

@@ -175,7 +175,7 @@ __device__ __forceinline__ void vbatch(int n, F f, ST st) {
 // With U > 1 the operands of 4 reactions are loaded before their products
 // (same products, same order); with SEG the Vdot sums go as flat segments.
 template <class M, int U = 1, bool SEG = false, class CY, class CF, class CR, class CA, class CV>
-__device__ void dev_fun(const CY &y, const CF &fix, const CR &rct, CA a, CV vdot) {
+__device__ __forceinline__ void dev_fun(const CY &y, const CF &fix, const CR &rct, CA a, CV vdot) {
   if constexpr (U > 1) {
     vbatch<4>(
         M::NREACT,
@@ -286,6 +286,57 @@ __device__ void dev_jac(const CY &y, const CF &fix, const CR &rct, CB b, CO out,
   }
 }
 
+// One LU position with at most 4 Jacobian terms and 4 Crout terms (81 % of
+// fullchem's positions): the Jacobian gather and the Crout sum of
+// dev_jac_decomp below, with every operand load issued before the first
+// wait (one memory round trip instead of two or three).  Same terms, same
+// order: g = sign * sum jv_c*b from 0 (gd on the diagonal), then
+// acc = fma(-L, U, acc), then the division.  The table entries are all
+// loaded before the guarded operand loads (lc_pin-style empty asm after
+// the last of them keeps them from being sunk into the guards).
+template <class M, class CB, class CL>
+__device__ __forceinline__ double lu_pos4(int q, bool isdiag, double gd, double sign, int dv, CB b,
+                                          CL lu) {
+  const int j0 = M::jv_ptr(q), jn = isdiag ? 0 : M::jv_ptr(q + 1) - j0;
+  const int c0 = M::cr_ptr(q), cn = M::cr_ptr(q + 1) - c0;
+  int jb_i[4], l_i[4], u_i[4];
+  double jc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int tj = min(j0 + u, M::NJV_TAB - 1), tc = min(c0 + u, M::NCR_TAB - 1);
+    jb_i[u] = M::jv_b(tj);
+    jc[u] = M::jv_c(tj);
+    l_i[u] = M::cr_l(tc);
+    u_i[u] = M::cr_u(tc);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) asm volatile("" ::"s"(jb_i[u]), "s"(l_i[u]), "s"(u_i[u]), "s"(jc[u]));
+  double jv[4], la[4], ub[4], dval = 1.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (u < jn) jv[u] = b[jb_i[u]];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (u < cn) {
+      la[u] = -(double)lu[l_i[u]];
+      ub[u] = lu[u_i[u]];
+    }
+  if (dv >= 0) dval = lu[dv];
+  double acc = gd;
+  if (!isdiag) {
+    double g = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (u < jn) g = fma(jc[u], jv[u], g);
+    acc = sign * g;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (u < cn) acc = fma(la[u], ub[u], acc);
+  if (dv >= 0) acc = acc / dval;
+  return acc;
+}
+
 // ros_PrepareMatrix's Ghimj = sign*JVS + diag (on the diagonal) and its
 // KppDecomp in one pass: each LU position starts from its Jacobian sum
 // (gathered from the partials) instead of a stored Ghimj, then takes the
@@ -310,11 +361,18 @@ __device__ int dev_jac_decomp(const CY &y, const CF &fix, const CR &rct, CB b, C
     const int q1 = M::lu_crow(k + 1);
 #pragma unroll 1
     for (; q < q1; ++q) {
+      const int dv = M::cr_div(q);
+      if constexpr (U > 1) {
+        if (M::cr_ptr(q + 1) - M::cr_ptr(q) <= 4 &&
+            (q == kd || M::jv_ptr(q + 1) - M::jv_ptr(q) <= 4)) {
+          lu[q] = lu_pos4<M>(q, q == kd, gd, sign, dv, b, lu);
+          continue;
+        }
+      }
       const double g = q == kd ? gd : ghimj(q);
       double acc = gather_fma<U>(M::cr_ptr(q), M::cr_ptr(q + 1), g,
                                  [&](int t) -> double { return -(double)lu[M::cr_l(t)]; },
                                  [&](int t) -> double { return lu[M::cr_u(t)]; });
-      const int dv = M::cr_div(q);
       if (dv >= 0) acc = acc / lu[dv];
       lu[q] = acc;
     }
@@ -424,6 +482,219 @@ __device__ void dev_solve(const CL &lu, CX x) {
     double acc = gather_fma<U>(kd + 1, k1, (double)x[i], [&](int k) -> double { return -(double)lu[k]; },
                                [&](int k) -> double { return x[M::lu_icol(k)]; });
     x[i] = acc / (double)lu[kd];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Operand cache of the persistent lane solver (the cached variants): CS
+// doubles per lane in LDS, slot s of a lane at lb[s * 64], plus a slot CS
+// that is written but never read.  Besides their tables the LU and the
+// substitution read a schedule computed offline (mechgen/lane_cache.py,
+// Belady's rule over the fixed access sequence), one descriptor byte per
+// operand read -- bits 5-6: 0 load from the slot block, 1 read LDS slot
+// (bits 0-4), 2 load and keep in the slot, 3 absent -- and one per result
+// (2: also keep it in the slot).  A gather goes in batches of up to 16
+// terms whose reads are all issued before the batch's fills, as the
+// schedule assumes.  Only where a value comes from changes: the sums, their
+// order and their rounding are those of the uncached walk.
+//
+// No branch per operand: every operand is one flat load whose address is
+// the lane's slot-block address plus a wave-uniform delta, the field offset
+// or the distance to the LDS slot; every operand is also written to LDS,
+// to its slot when it is kept, else to slot CS.
+struct LcLane {
+  uint64_t ga;  // the lane's element 0 of its slot block (generic address)
+  uint64_t dl;  // (generic address of the lane's LDS slot 0) - ga: the same for all lanes
+  double *lb;   // the lane's LDS slot 0
+};
+
+// Byte i of a record's words (i is a constant once the loops are unrolled).
+__device__ __forceinline__ unsigned lc_byte(const unsigned *w, int i) {
+  return (w[i >> 2] >> ((i & 3) * 8)) & 255u;
+}
+template <int CS>
+__device__ __forceinline__ double lc_rd(const LcLane &c, unsigned d, int soff) {
+  const uint64_t delta = (d >> 5) == 1u ? c.dl + (uint64_t)((d & 31u) * 512u)
+                                        : (uint64_t)(uint32_t)soff;
+  return *(const double *)(c.ga + delta);
+}
+template <int CS>
+__device__ __forceinline__ void lc_wr(const LcLane &c, unsigned d, double v) {
+  const unsigned s = (d >> 5) == 2u ? (d & 31u) : (unsigned)CS;
+  c.lb[s * 64] = v;
+}
+// LDS writes complete before the next batch's flat loads read them (flat
+// loads and LDS instructions are not ordered with each other).
+__device__ __forceinline__ void lc_fence() { __builtin_amdgcn_s_waitcnt(0xc07f); }
+
+// One batch of R terms of acc = fma(c(t), v(t), acc): a record of R bytes
+// (v's operand).  Returns the next record's word.
+template <int CS, int R, class DW, class FC, class FV>
+__device__ __forceinline__ int lc_batch1(DW dw, int dp, const LcLane &c, int t, double &acc, FC cm,
+                                         FV vm) {
+  constexpr int NW = (R + 3) / 4;
+  lc_fence();
+  unsigned w[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) w[k] = dw(dp + k);
+  double v[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) v[u] = lc_rd<CS>(c, lc_byte(w, u), vm(t + u).soff);
+#pragma unroll
+  for (int u = 0; u < R; ++u) lc_wr<CS>(c, lc_byte(w, u), v[u]);
+#pragma unroll
+  for (int u = 0; u < R; ++u) acc = fma(cm(t + u), v[u], acc);
+  return dp + NW;
+}
+
+// acc = 0; acc = fma(c(t), v(t), acc) over [t, t1) in batches of 16 while
+// 16 remain, then of 8, 4, 2, 1 (the sizes the schedule assumes).
+template <int CS, class DW, class FC, class FV>
+__device__ __forceinline__ double lc_sum1(DW dw, int &dp, const LcLane &c, int t, const int t1,
+                                          FC cm, FV vm) {
+  double acc = 0.0;
+#pragma unroll 1
+  for (; t + 16 <= t1; t += 16) dp = lc_batch1<CS, 16>(dw, dp, c, t, acc, cm, vm);
+  if (t + 8 <= t1) { dp = lc_batch1<CS, 8>(dw, dp, c, t, acc, cm, vm); t += 8; }
+  if (t + 4 <= t1) { dp = lc_batch1<CS, 4>(dw, dp, c, t, acc, cm, vm); t += 4; }
+  if (t + 2 <= t1) { dp = lc_batch1<CS, 2>(dw, dp, c, t, acc, cm, vm); t += 2; }
+  if (t < t1) dp = lc_batch1<CS, 1>(dw, dp, c, t, acc, cm, vm);
+  return acc;
+}
+
+// Extra operands and result of a two-operand sum (its first record's header
+// [x0, result, x2, -]).
+struct LcHead {
+  bool hx0, hx2, init_x0;
+  double init, x0v, x2v;
+  unsigned res;
+};
+
+// One batch of R terms of acc = fma(-a(t), b(t), acc): a record of 2R bytes
+// (a, b per term), after the header when HDR (the sum's first batch, which
+// also reads the extra operands and starts acc).  Returns the next record.
+template <int CS, int R, bool HDR, class DW, class FX0, class FX2, class FA, class FB>
+__device__ __forceinline__ int lc_batch2(DW dw, int dp, const LcLane &c, int t, double &acc,
+                                         LcHead &h, FX0 x0m, FX2 x2m, FA am, FB bm) {
+  constexpr int B0 = HDR ? 4 : 0;
+  constexpr int NW = (B0 + 2 * R + 3) / 4;
+  constexpr int RA = R > 0 ? R : 1;
+  lc_fence();
+  unsigned w[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) w[k] = dw(dp + k);
+  double a[RA], b[RA];
+  if constexpr (HDR) {
+    h.res = lc_byte(w, 1);
+    if (h.hx0) h.x0v = lc_rd<CS>(c, lc_byte(w, 0), x0m().soff);
+    if (h.hx2) h.x2v = lc_rd<CS>(c, lc_byte(w, 2), x2m().soff);
+  }
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    a[u] = lc_rd<CS>(c, lc_byte(w, B0 + 2 * u), am(t + u).soff);
+    b[u] = lc_rd<CS>(c, lc_byte(w, B0 + 2 * u + 1), bm(t + u).soff);
+  }
+  if constexpr (HDR) {
+    if (h.hx0) lc_wr<CS>(c, lc_byte(w, 0), h.x0v);
+    if (h.hx2) lc_wr<CS>(c, lc_byte(w, 2), h.x2v);
+    acc = h.init_x0 ? h.x0v : h.init;
+  }
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    lc_wr<CS>(c, lc_byte(w, B0 + 2 * u), a[u]);
+    lc_wr<CS>(c, lc_byte(w, B0 + 2 * u + 1), b[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < R; ++u) acc = fma(-a[u], b[u], acc);
+  return dp + NW;
+}
+
+// acc = (h.init_x0 ? x0 : h.init); acc = fma(-a(t), b(t), acc) over
+// [t, t1), batches as in lc_sum1 (the first carries the header).
+template <int CS, class DW, class FX0, class FX2, class FA, class FB>
+__device__ __forceinline__ double lc_sum2(DW dw, int &dp, const LcLane &c, int t, const int t1,
+                                          LcHead &h, FX0 x0m, FX2 x2m, FA am, FB bm) {
+  double acc = 0.0;
+  const int n = t1 - t;
+  if (n >= 16) { dp = lc_batch2<CS, 16, true>(dw, dp, c, t, acc, h, x0m, x2m, am, bm); t += 16; }
+  else if (n >= 8) { dp = lc_batch2<CS, 8, true>(dw, dp, c, t, acc, h, x0m, x2m, am, bm); t += 8; }
+  else if (n >= 4) { dp = lc_batch2<CS, 4, true>(dw, dp, c, t, acc, h, x0m, x2m, am, bm); t += 4; }
+  else if (n >= 2) { dp = lc_batch2<CS, 2, true>(dw, dp, c, t, acc, h, x0m, x2m, am, bm); t += 2; }
+  else if (n == 1) { dp = lc_batch2<CS, 1, true>(dw, dp, c, t, acc, h, x0m, x2m, am, bm); t += 1; }
+  else dp = lc_batch2<CS, 0, true>(dw, dp, c, t, acc, h, x0m, x2m, am, bm);
+#pragma unroll 1
+  for (; t + 16 <= t1; t += 16)
+    dp = lc_batch2<CS, 16, false>(dw, dp, c, t, acc, h, x0m, x2m, am, bm);
+  if (t + 8 <= t1) { dp = lc_batch2<CS, 8, false>(dw, dp, c, t, acc, h, x0m, x2m, am, bm); t += 8; }
+  if (t + 4 <= t1) { dp = lc_batch2<CS, 4, false>(dw, dp, c, t, acc, h, x0m, x2m, am, bm); t += 4; }
+  if (t + 2 <= t1) { dp = lc_batch2<CS, 2, false>(dw, dp, c, t, acc, h, x0m, x2m, am, bm); t += 2; }
+  if (t < t1) dp = lc_batch2<CS, 1, false>(dw, dp, c, t, acc, h, x0m, x2m, am, bm);
+  return acc;
+}
+
+// dev_jac_decomp through the operand cache (same values, same order).
+template <class M, int CS, class CY, class CF, class CR, class CB, class CL>
+__device__ __forceinline__ int dev_jac_decomp_c(const CY &y, const CF &fix, const CR &rct, CB b, CL lu,
+                                double sign, double diag, const LcLane &c) {
+  dev_partials<M, 16>(y, fix, rct, b);
+  auto dw = [&](int i) { return M::lc_lu(CS, i); };
+  int dp = 0;
+  auto ghimj = [&](int q) -> double {
+    return sign * lc_sum1<CS>(dw, dp, c, M::jv_ptr(q), M::jv_ptr(q + 1),
+                          [&](int t) -> double { return M::jv_c(t); },
+                          [&](int t) { return b[M::jv_b(t)]; });
+  };
+  int ier = 0;
+  int q = 0;
+#pragma unroll 1
+  for (int k = 0; k < M::NVAR; ++k) {
+    const int kd = M::lu_diag(k);
+    const double gd = ghimj(kd) + diag;
+    if (ier == 0 && fabs(gd) < DBL_MIN) ier = k + 1;
+    const int q1 = M::lu_crow(k + 1);
+#pragma unroll 1
+    for (; q < q1; ++q) {
+      const double g = q == kd ? gd : ghimj(q);
+      const int dv = M::cr_div(q);
+      LcHead h{dv >= 0, false, false, g, 0.0, 0.0, 0u};
+      double acc = lc_sum2<CS>(dw, dp, c, M::cr_ptr(q), M::cr_ptr(q + 1), h,
+                           [&]() { return lu[dv]; }, [&]() { return lu[0]; },
+                           [&](int t) { return lu[M::cr_l(t)]; },
+                           [&](int t) { return lu[M::cr_u(t)]; });
+      if (dv >= 0) acc = acc / h.x0v;
+      lu[q] = acc;
+      lc_wr<CS>(c, h.res, acc);
+    }
+  }
+  return ier;
+}
+
+// dev_solve through the operand cache (same values, same order).
+template <class M, int CS, class CL, class CX>
+__device__ __forceinline__ void dev_solve_c(const CL &lu, CX x, const LcLane &c) {
+  auto dw = [&](int i) { return M::lc_sv(CS, i); };
+  int dp = 0;
+#pragma unroll 1
+  for (int i = 0; i < M::NVAR; ++i) {
+    const int k0 = M::lu_crow(i), k1 = M::lu_diag(i);
+    if (k0 == k1) continue;
+    LcHead h{true, false, true, 0.0, 0.0, 0.0, 0u};
+    const double acc = lc_sum2<CS>(dw, dp, c, k0, k1, h, [&]() { return x[i]; },
+                               [&]() { return x[i]; }, [&](int k) { return lu[k]; },
+                               [&](int k) { return x[M::lu_icol(k)]; });
+    x[i] = acc;
+    lc_wr<CS>(c, h.res, acc);
+  }
+#pragma unroll 1
+  for (int i = M::NVAR - 1; i >= 0; --i) {
+    const int kd = M::lu_diag(i), k1 = M::lu_crow(i + 1);
+    LcHead h{true, true, true, 0.0, 0.0, 0.0, 0u};
+    double acc = lc_sum2<CS>(dw, dp, c, kd + 1, k1, h, [&]() { return x[i]; },
+                         [&]() { return lu[kd]; }, [&](int k) { return lu[k]; },
+                         [&](int k) { return x[M::lu_icol(k)]; });
+    acc = acc / h.x2v;
+    x[i] = acc;
+    lc_wr<CS>(c, h.res, acc);
   }
 }
 
@@ -653,9 +924,9 @@ __device__ __forceinline__ void lane_begin(const IntegrateArgs &a, const Slot &s
 
 // One Rodas3 attempt (or the end-of-integration checks); true when the cell
 // is done (S.ierr then holds IERR).
-template <class M, int U, bool SEG>
+template <class M, int U, bool SEG, int CS = 0>
 __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, const Slot &sl, int64_t cell,
-                                             LaneState &S) {
+                                             LaneState &S, const LcLane &lc = LcLane{}) {
   using L = SlotLayout<M>;
   const int N = M::NVAR;
   constexpr int VB = U > 1 ? 4 : 1;
@@ -684,6 +955,9 @@ __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, const Slot 
     if constexpr (SEG) {
       dev_jac<M, U, SEG>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * S.H * kRosGamma1));
       ising = dev_decomp_crout<M, U, SEG>(LU);
+    } else if constexpr (CS > 0) {
+      ising = dev_jac_decomp_c<M, CS>(Yc, Yc, rct, Ascr, LU, -1.0,
+                                      1.0 / (dir * S.H * kRosGamma1), lc);
     } else {
       ising = dev_jac_decomp<M, U>(Yc, Yc, rct, Ascr, LU, -1.0, 1.0 / (dir * S.H * kRosGamma1));
     }
@@ -726,7 +1000,8 @@ __device__ __forceinline__ bool lane_attempt(const IntegrateArgs &a, const Slot 
                    [&](int i, double v) { Kc[i] = v; });
       }
     }
-    dev_solve<M, U, SEG>(LU, Kc);
+    if constexpr (CS > 0) dev_solve_c<M, CS>(LU, Kc, lc);
+    else dev_solve<M, U, SEG>(LU, Kc);
     S.st[6] += 1;
   }
   double err = 0.0;
@@ -870,17 +1145,27 @@ template <> struct IsEuler<Traits_carbon> : std::true_type {};
 
 // Up to three queues M0, M1, M2 (void: absent); U: loads in flight per
 // gather (see gather_fma); W: waves per SIMD the register allocation must
-// allow (1: unconstrained).
+// allow (1: unconstrained); CS: operand-cache slots per lane (0: none; the
+// launch provides blockDim.x * (CS + 1) doubles of dynamic LDS).
 // A workgroup holds blockDim.x/64 waves (cells per workgroup = blockDim.x);
 // each wave has its own slot block.
-template <int U, int W, bool SEG, class M0, class M1, class M2>
+template <int U, int W, bool SEG, int CS, class M0, class M1, class M2>
 __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) +
-                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const Slot sl{__builtin_amdgcn_make_buffer_rsrc(pa.work + wave * (64 * pa.slot_fields), (short)0,
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+  double *const wblock = pa.work + wave * (64 * pa.slot_fields);
+  const Slot sl{__builtin_amdgcn_make_buffer_rsrc(wblock, (short)0,
                                                   (int)(64 * pa.slot_fields * 8), 0x00020000),
                 lane * 8};
+  extern __shared__ double lc_lds[];
+  LcLane lc{};
+  if constexpr (CS > 0) {
+    double *const lwave = lc_lds + wib * ((CS + 1) * 64);
+    lc.ga = (uint64_t)(wblock + lane);
+    lc.dl = (uint64_t)lwave - (uint64_t)wblock;
+    lc.lb = lwave + lane;
+  }
   constexpr int NQ = std::is_void_v<M1> ? 1 : (std::is_void_v<M2> ? 2 : 3);
   const int64_t total = pa.qend[NQ - 1];
   int64_t cell = -1;
@@ -911,7 +1196,7 @@ __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
         lane_euler<M>(pa.q[q], sl, S);
         done = true;
       } else {
-        done = lane_attempt<M, U, SEG>(pa.q[q], sl, cell, S);
+        done = lane_attempt<M, U, SEG, CS>(pa.q[q], sl, cell, S, lc);
       }
     });
     if (done) {
@@ -1035,19 +1320,20 @@ void launch_integrate_mech(int mech, const IntegrateArgs &a, hipStream_t s) {
 
 // Persistent-launch variants: queue sets (one mechanism, or fullchem, Hg
 // and carbon in that order with any queue possibly empty) x U x W.
-template <int U, int W, bool SEG>
+template <int U, int W, bool SEG, int CS = 0>
 static const void *persist_kernel(int qset) {
   switch (qset) {
-    case PQ_FULLCHEM: return (const void *)k_persist<U, W, SEG, Traits_fullchem, void, void>;
-    case PQ_HG: return (const void *)k_persist<U, W, SEG, Traits_Hg, void, void>;
-    case PQ_CARBON: return (const void *)k_persist<U, W, SEG, Traits_carbon, void, void>;
+    case PQ_FULLCHEM: return (const void *)k_persist<U, W, SEG, CS, Traits_fullchem, void, void>;
+    case PQ_HG: return (const void *)k_persist<U, W, SEG, CS, Traits_Hg, void, void>;
+    case PQ_CARBON: return (const void *)k_persist<U, W, SEG, CS, Traits_carbon, void, void>;
     case PQ_MIXED:
-      return (const void *)k_persist<U, W, SEG, Traits_fullchem, Traits_Hg, Traits_carbon>;
+      return (const void *)k_persist<U, W, SEG, CS, Traits_fullchem, Traits_Hg, Traits_carbon>;
   }
   return nullptr;
 }
-// (U, W, SEG): terms per batched gather, minimum waves per SIMD (register
-// budget), flat segment execution of the independent sums
+// (U, W, SEG, CS): terms per batched gather, minimum waves per SIMD
+// (register budget), flat segment execution of the independent sums,
+// operand-cache slots per lane
 const void *persist_kernel_variant(int qset, int variant) {
   switch (variant) {
     case 0: return persist_kernel<1, 1, false>(qset);
@@ -1058,11 +1344,15 @@ const void *persist_kernel_variant(int qset, int variant) {
     case 5: return persist_kernel<16, 4, true>(qset);
     case 6: return persist_kernel<8, 5, false>(qset);
     case PERSIST_WAVE_VARIANT: return wave_kernel(qset);
+    case 8: return persist_kernel<16, 4, false, 16>(qset);
+    case 9: return persist_kernel<16, 2, false, 32>(qset);
   }
   return nullptr;
 }
+int persist_variant_cache(int variant) { return variant == 8 ? 16 : variant == 9 ? 32 : 0; }
 int persist_variant_info(int variant, int *U, int *W) {
-  static const int tab[][2] = {{1, 1}, {8, 1}, {8, 4}, {8, 4}, {16, 4}, {16, 4}, {8, 5}, {0, 0}};
+  static const int tab[][2] = {{1, 1}, {8, 1}, {8, 4}, {8, 4}, {16, 4}, {16, 4}, {8, 5}, {0, 0},
+                               {16, 4}, {16, 2}};
   if (variant < 0 || variant >= PERSIST_VARIANTS) return -1;
   *U = tab[variant][0];
   *W = tab[variant][1];
@@ -1093,7 +1383,9 @@ int64_t persist_resident_waves(int qset, int variant, int wpb) {
   if (!k || hipGetDevice(&dev) != hipSuccess) return 0;
   if (variant == PERSIST_WAVE_VARIANT && wpb != 1) return 0;  // its workgroup is one wave
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64 * wpb, 0) != hipSuccess) return 0;
+  const int cs = persist_variant_cache(variant);
+  const size_t lds = cs ? (size_t)64 * wpb * (cs + 1) * sizeof(double) : 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64 * wpb, lds) != hipSuccess) return 0;
   return (int64_t)ncu * (per > 0 ? per : 1) * wpb;
 }
 // nwaves waves in workgroups of wpb waves; counter null: one cell per lane
@@ -1106,7 +1398,10 @@ int launch_persist(int qset, int variant, int wpb, const PersistArgs &pa, int64_
   if (variant == PERSIST_WAVE_VARIANT && wpb != 1) return -1;
   if (pa.counter) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.counter);
   void *args[] = {(void *)&pa};
-  return hipLaunchKernel(k, dim3((unsigned)(nwaves / wpb)), dim3(64 * wpb), args, 0, s) == hipSuccess
+  const int cs = persist_variant_cache(variant);
+  const size_t lds = cs ? (size_t)64 * wpb * (cs + 1) * sizeof(double) : 0;
+  return hipLaunchKernel(k, dim3((unsigned)(nwaves / wpb)), dim3(64 * wpb), args, lds, s) ==
+                 hipSuccess
              ? 0 : -1;
 }
 void launch_comp_mech(int mech, int which, const CompArgs &a, hipStream_t s) {
