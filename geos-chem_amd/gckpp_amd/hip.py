@@ -32,7 +32,7 @@ SYMBOLS = [
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
-PERSIST_VARIANTS = 8        # gckpp_persistent_variants(); checked by the CPU tests
+PERSIST_VARIANTS = 10       # gckpp_persistent_variants(); checked by the CPU tests
 WAVE_VARIANT = 7            # one wavefront per cell, state in LDS (64-thread workgroups)
 
 
@@ -188,13 +188,14 @@ class Context:
 
     def persistent_config(self):
         """dict(variant, waves_per_group, waves, loads_in_flight,
-        min_waves_per_simd, segments, wave_per_cell)."""
-        out = (ctypes.c_int32 * 6)()
+        min_waves_per_simd, segments, wave_per_cell, cache_slots)."""
+        out = (ctypes.c_int32 * 7)()
         _check(self.L.gckpp_get_persistent_config(self.h, out), "gckpp_get_persistent_config")
         d = dict(zip(("variant", "waves_per_group", "waves", "loads_in_flight",
                       "min_waves_per_simd"), list(out)[:5]))
         d["segments"] = int(out[5] == 1)
         d["wave_per_cell"] = int(out[5] == 2)
+        d["cache_slots"] = int(out[6])
         return d
 
     def autotune(self, conc, rconst, hstart, tin, tout, icntrl, rcntrl):

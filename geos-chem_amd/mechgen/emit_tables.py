@@ -20,6 +20,8 @@ wavefront-uniform index, so the compiler reads it through the scalar cache.
                         for the persistent solver's batched execution
   wv_*, WV_*            the cell-per-wave solver's programs and LDS layout
                         (wave_sched.py)
+  lc<S>_lu, lc<S>_sv    the persistent solver's operand-cache schedules with
+                        S slots per lane (lane_cache.py)
 Run:  python emit_tables.py <mech> [<mech> ...]
 """
 import json
@@ -27,6 +29,7 @@ import sys
 from pathlib import Path
 
 sys.path.insert(0, str(Path(__file__).resolve().parent))
+import lane_cache  # noqa: E402
 import wave_sched  # noqa: E402
 
 HERE = Path(__file__).resolve().parent
@@ -240,6 +243,13 @@ def emit(name):
         if P not in have:
             for suf, ctype in (("lev", "int"), ("rs", "int"), ("head", "unsigned"), ("term", "unsigned")):
                 out.append(arr(ctype, "wv_%s_%s" % (P, suf), [0]))
+    # persistent lane solver: operand-cache schedules of the LU and
+    # substitution sections (lane_cache.py), 16 pad words at the end so a
+    # record's words can be loaded whole
+    for S in lane_cache.SLOTS:
+        lw, sw = lane_cache.build(m, t, S)
+        out.append(arr("unsigned", "lc%d_lu" % S, lw + [0] * 16))
+        out.append(arr("unsigned", "lc%d_sv" % S, sw + [0] * 16))
     out.append("}  // namespace mech_%s" % name)
     gen = HERE.parent / "csrc" / "gen"
     gen.mkdir(exist_ok=True)

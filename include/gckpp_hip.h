@@ -132,12 +132,15 @@ int gckpp_step_timings(gckpp_ctx *ctx, float ms[4]);
  * independent rows) as flat segment lists: (1,1,0) (8,1,0) (8,4,0) (8,4,1)
  * (16,4,0) (16,4,1) (8,5,0); waves_per_group 1, 2 or 4 (cells per
  * workgroup 64, 128, 256).  Variant 7: one wavefront per cell with the
- * cell's state in LDS (waves_per_group must be 1; U = W = 0, S = 2).  All
- * variants add the same terms in the same order.  get: out = {variant,
- * waves_per_group, waves per launch, U, W, S}. */
+ * cell's state in LDS (waves_per_group must be 1; U = W = 0, S = 2).
+ * Variants 8, 9: one lane per cell with an operand cache of C = 16 / 32
+ * values per lane in LDS for the LU and the substitutions, managed by a
+ * schedule computed when the mechanism is compiled ((16,4,0), (16,2,0)).
+ * All variants add the same terms in the same order.  get: out = {variant,
+ * waves_per_group, waves per launch, U, W, S, C}. */
 int gckpp_persistent_variants(void);
 int gckpp_set_persistent_config(gckpp_ctx *ctx, int variant, int waves_per_group);
-int gckpp_get_persistent_config(gckpp_ctx *ctx, int32_t out[6]);
+int gckpp_get_persistent_config(gckpp_ctx *ctx, int32_t out[7]);
 
 /* Persistent solver extras (no reference counterpart).  order[ncell]
  * (device, a permutation of 0..ncell-1, or null): the order in which the

@@ -55,7 +55,8 @@ def box2k(fullchem):
     return dict(w=w, rct=rct, ic=ic, rc=rc, c=c_o, ist=i_o, ierr=e_o)
 
 
-@pytest.mark.parametrize("variant,wpb", [(0, 1), (1, 2), (2, 1), (3, 1), (4, 4), (5, 2), (6, 1), (7, 1)])
+@pytest.mark.parametrize("variant,wpb", [(0, 1), (1, 2), (2, 1), (3, 1), (4, 4), (5, 2), (6, 1), (7, 1),
+                                         (8, 4), (9, 2)])
 def test_persistent_variants_match_oracle(fullchem, torch_dev, box2k, variant, wpb):
     print("case: persistent variant %d, %d waves per workgroup, 2000 box cells" % (variant, wpb),
           flush=True)
@@ -221,16 +222,19 @@ def test_mixed_batch_equals_per_mechanism_steps_and_graph_replays(fullchem, hg, 
         c.close()
 
 
+@pytest.mark.parametrize("variant,wpb", [(7, 1), (8, 1), (8, 4), (9, 2)])
 @pytest.mark.parametrize("name,forced", [("fullchem", False), ("fullchem", True), ("Hg", False),
                                          ("carbon", False)])
-def test_wave_per_cell_step_bitwise_equals_lane_per_cell(name, forced, mech_of, torch_dev):
+def test_step_bitwise_equals_lane_per_cell(name, forced, variant, wpb, mech_of, torch_dev):
     """The cell-per-wave solver (state in LDS, sums shared by the wave's
-    lanes) performs the lane-per-cell solver's operations in the same order:
-    the fused step (rates, Integrate, retry, clip, next Hstart) gives bitwise
-    the same concentrations, ISTATUS, IERR and Hstart.  forced: Hstart 1e-13 s
-    and ICNTRL(4) = 40 make many cells fail and take the retry."""
-    print("case: wave-per-cell vs lane-per-cell step, %s, forced retries %s" % (name, forced),
-          flush=True)
+    lanes) and the lane solver with the LDS operand cache perform the plain
+    lane-per-cell solver's operations in the same order: the fused step
+    (rates, Integrate, retry, clip, next Hstart) gives bitwise the same
+    concentrations, ISTATUS, IERR and Hstart.  forced: Hstart 1e-13 s and
+    ICNTRL(4) = 40 make many cells fail and take the retry (the cached
+    solver then restarts its schedules mid-cell)."""
+    print("case: variant %d (%d waves per group) vs lane-per-cell step, %s, forced retries %s"
+          % (variant, wpb, name, forced), flush=True)
     from gckpp_amd import hip, synth
     from oracle.ref import icntrl_for
     T = torch_dev
@@ -243,17 +247,19 @@ def test_wave_per_cell_step_bitwise_equals_lane_per_cell(name, forced, mech_of, 
         ic[3] = 40
         hs = np.full(w.ncell, 1e-13)
     out = {}
-    for variant in (4, hip.WAVE_VARIANT):
+    for v, g in ((4, 1), (variant, wpb)):
         ctx = hip.Context(m, 2048)
         ctx.set_solver(hip.SOLVER_PERSISTENT)
-        ctx.set_persistent_config(variant, 1)
-        assert ctx.persistent_config()["wave_per_cell"] == (variant == hip.WAVE_VARIANT)
+        ctx.set_persistent_config(v, g)
+        cfg = ctx.persistent_config()
+        assert cfg["wave_per_cell"] == (v == hip.WAVE_VARIANT)
+        assert cfg["cache_slots"] == {8: 16, 9: 32}.get(v, 0)
         conc = _t(T, w.conc)
         hnew, ist, ierr, ok = ctx.fullchem_step(_t(T, w.met), _t(T, w.photol), _t(T, w.het), conc,
                                                 _t(T, hs), 1200.0, ic, rc)
-        out[variant] = (conc.cpu().numpy(), hnew.cpu().numpy(), ist.cpu().numpy(), ierr.cpu().numpy())
+        out[v] = (conc.cpu().numpy(), hnew.cpu().numpy(), ist.cpu().numpy(), ierr.cpu().numpy())
         ctx.close()
-    a, b = out[4], out[hip.WAVE_VARIANT]
+    a, b = out[4], out[variant]
     if forced:
         print("failed cells", int((a[3] < 0).sum()), "of", w.ncell)
         assert (a[3] < 0).sum() > 20 and (a[3] == 1).sum() > 20
@@ -261,12 +267,13 @@ def test_wave_per_cell_step_bitwise_equals_lane_per_cell(name, forced, mech_of, 
         assert np.array_equal(x, y)
 
 
-def test_mixed_batch_wave_per_cell_bitwise_and_graph_replays(fullchem, hg, carbon, torch_dev):
-    """The mixed-mechanism batch with the cell-per-wave kernel: one step
-    bitwise equal to the lane-per-cell batch step and to each mechanism's
-    own persistent step; a captured graph replayed three times equals three
-    eager steps."""
-    print("case: mixed batch with the wave-per-cell kernel, graph replay", flush=True)
+@pytest.mark.parametrize("variant", [7, 8, 9])
+def test_mixed_batch_bitwise_and_graph_replays(variant, fullchem, hg, carbon, torch_dev):
+    """The mixed-mechanism batch with the cell-per-wave kernel (7) and the
+    cached lane kernels (8, 9): one step bitwise equal to the plain
+    lane-per-cell batch step and to each mechanism's own persistent step; a
+    captured graph replayed three times equals three eager steps."""
+    print("case: mixed batch with persistent variant %d, graph replay" % variant, flush=True)
     from gckpp_amd import hip
     T = torch_dev
     inputs = _mixed_inputs(fullchem, hg, carbon)
@@ -283,18 +290,18 @@ def test_mixed_batch_wave_per_cell_bitwise_and_graph_replays(fullchem, hg, carbo
         return st, [B.part(icntrl=inp[2], rcntrl=inp[3], **d) for inp, d in zip(inputs, st)]
 
     res = {}
-    for variant in (4, hip.WAVE_VARIANT):
-        B.set_config(variant, 1)
+    for v in (4, variant):
+        B.set_config(v, 1)
         st, parts = make_state()
         B.step(parts, 1200.0)
         T.cuda.synchronize()
-        res[variant] = [(d["conc"].cpu().numpy(), d["hstart"].cpu().numpy(),
-                         d["istatus"].cpu().numpy(), d["ierr"].cpu().numpy()) for d in st]
-    for ra, rb in zip(res[4], res[hip.WAVE_VARIANT]):
+        res[v] = [(d["conc"].cpu().numpy(), d["hstart"].cpu().numpy(),
+                   d["istatus"].cpu().numpy(), d["ierr"].cpu().numpy()) for d in st]
+    for ra, rb in zip(res[4], res[variant]):
         for x, y in zip(ra, rb):
             assert np.array_equal(x, y)
     # each mechanism alone through its own context's persistent step
-    for ctx, (m, w, ic, rc), rb in zip(ctxs, inputs, res[hip.WAVE_VARIANT]):
+    for ctx, (m, w, ic, rc), rb in zip(ctxs, inputs, res[variant]):
         ctx.set_solver(hip.SOLVER_PERSISTENT)
         ctx.set_persistent_config(4, 1)
         conc = _t(T, w.conc)
@@ -302,7 +309,7 @@ def test_mixed_batch_wave_per_cell_bitwise_and_graph_replays(fullchem, hg, carbo
                                                 _t(T, w.hstart), 1200.0, ic, rc)
         assert np.array_equal(conc.cpu().numpy(), rb[0]), m.name
         assert np.array_equal(hnew.cpu().numpy(), rb[1]), m.name
-    B.set_config(hip.WAVE_VARIANT, 1)
+    B.set_config(variant, 1)
     st_e, parts_e = make_state()
     for _ in range(3):
         B.step(parts_e, 1200.0)
