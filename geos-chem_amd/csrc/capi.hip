@@ -98,15 +98,25 @@ static int set_pconfig(gckpp_ctx *c, int variant, int wpb) {
   return 0;
 }
 
+// A failed gckpp_init releases what it allocated before reporting.
+#define INIT_OK(x)                                                             \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      gckpp_finalize(c);                                                       \
+      return fail("gckpp_init: %s failed: %s", #x, hipGetErrorString(e_));     \
+    }                                                                          \
+  } while (0)
+
 extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   if (!out) return fail("gckpp_init: null output pointer");
   *out = nullptr;
   if (!mech_ok(mech_id)) return fail("gckpp_init: unknown or unsupported mechanism id %d", mech_id);
   if (max_cells <= 0) return fail("gckpp_init: max_cells must be positive");
-  auto *c = new gckpp_ctx();
+  auto *c = new gckpp_ctx();  // value-initialised: every pointer and event null
   c->mech = mech_id;
   c->info = mech_info(mech_id);
-  HIP_OK(hipGetDevice(&c->device));
+  INIT_OK(hipGetDevice(&c->device));
   c->max_cells = max_cells;
   c->wstride = (max_cells + 63) / 64 * 64;
   int64_t per_cell = integrate_work_per_cell(c->info);
@@ -114,24 +124,28 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   size_t wbytes = sizeof(double) * (size_t)per_cell * (size_t)c->wstride;
   hipError_t e = hipMalloc(&c->work, wbytes);
   if (e != hipSuccess) {
-    delete c;
+    c->work = nullptr;
+    gckpp_finalize(c);
     return fail("gckpp_init: cannot allocate %.2f GB of workspace for %lld cells: %s",
                 wbytes / 1e9, (long long)max_cells, hipGetErrorString(e));
   }
-  HIP_OK(hipMalloc(&c->atol, sizeof(double) * c->info.nvar));
-  HIP_OK(hipMalloc(&c->rtol, sizeof(double) * c->info.nvar));
-  HIP_OK(hipMalloc(&c->consts, sizeof(double) * 3 * c->info.nspec));
-  HIP_OK(hipMemset(c->consts, 0, sizeof(double) * 3 * c->info.nspec));
+  INIT_OK(hipMalloc(&c->atol, sizeof(double) * c->info.nvar));
+  INIT_OK(hipMalloc(&c->rtol, sizeof(double) * c->info.nvar));
+  INIT_OK(hipMalloc(&c->consts, sizeof(double) * 3 * c->info.nspec));
+  INIT_OK(hipMemset(c->consts, 0, sizeof(double) * 3 * c->info.nspec));
   {
     size_t sb = sizeof(double) * (size_t)(c->info.nreact + c->info.nspec + 20) * (size_t)c->wstride;
     hipError_t e2 = hipMalloc(&c->step_buf, sb);
-    if (e2 != hipSuccess)
+    if (e2 != hipSuccess) {
+      c->step_buf = nullptr;
+      gckpp_finalize(c);
       return fail("gckpp_init: cannot allocate %.2f GB of step buffers: %s", sb / 1e9,
                   hipGetErrorString(e2));
+    }
   }
-  HIP_OK(hipMalloc(&c->nfail, sizeof(int32_t)));
-  for (int k = 0; k < 5; ++k) HIP_OK(hipEventCreate(&c->ev[k]));
-  HIP_OK(hipMalloc(&c->counter, sizeof(unsigned int)));
+  INIT_OK(hipMalloc(&c->nfail, sizeof(int32_t)));
+  for (int k = 0; k < 5; ++k) INIT_OK(hipEventCreate(&c->ev[k]));
+  INIT_OK(hipMalloc(&c->counter, sizeof(unsigned int)));
   c->solver = GCKPP_SOLVER_PLAIN;
   if (set_pconfig(c, kDefaultPersistVariant, kDefaultPersistWpb) != 0) {
     c->pvariant = kDefaultPersistVariant;
@@ -141,8 +155,8 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   // Init_FullChem defaults (GeosCore/fullchem_mod.F90:2899-2905)
   c->h_atol.assign(c->info.nvar, 1.0e-2);
   c->h_rtol.assign(c->info.nvar, 0.5e-2);
-  HIP_OK(hipMemcpy(c->atol, c->h_atol.data(), sizeof(double) * c->info.nvar, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(c->rtol, c->h_rtol.data(), sizeof(double) * c->info.nvar, hipMemcpyHostToDevice));
+  INIT_OK(hipMemcpy(c->atol, c->h_atol.data(), sizeof(double) * c->info.nvar, hipMemcpyHostToDevice));
+  INIT_OK(hipMemcpy(c->rtol, c->h_rtol.data(), sizeof(double) * c->info.nvar, hipMemcpyHostToDevice));
   *out = c;
   g_err[0] = 0;
   return 0;
@@ -539,8 +553,11 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
     if (code == -100) return fail("gckpp_fullchem_step: %s", why);
     HIP_OK(hipEventRecord(c->ev[2], s));
     if (code < 0) {
+      // every cell fails the same option check (twice, as the retry would):
+      // IERR, zero ISTATUS and RSTATUS, then the clip and Hnew = RSTATUS(3)
+      // below, as in the plain step
       hipLaunchKernelGGL(k_fill_status, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s,
-                         ncell, ierr, code, istatus, (double *)nullptr);
+                         ncell, ierr, code, istatus, rst);
     } else {
       PersistArgs pa{};
       fill_persist_queue(c, pa.q[0], ncell, p, conc, rct, hstart, hnew, istatus, rst, ierr);
@@ -561,8 +578,10 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
     HIP_OK(hipEventRecord(c->ev[3], s));
     int64_t nb = (ncell + 255) / 256;
     int32_t *partial = (int32_t *)cbak;
+    const bool opt_err = code < 0;  // else the persistent kernel clipped and wrote Hnew
     hipLaunchKernelGGL(k_step_finish, dim3((unsigned)nb), dim3(256), 0, s, ncell,
-                       c->info.nspec, 0, conc, (const double *)nullptr, (double *)nullptr, ierr,
+                       c->info.nspec, opt_err && c->mech != MECH_CARBON ? 1 : 0, conc,
+                       opt_err ? (const double *)rst : nullptr, opt_err ? hnew : nullptr, ierr,
                        partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, partial, c->nfail);
     HIP_OK(hipGetLastError());

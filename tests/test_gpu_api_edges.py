@@ -162,3 +162,42 @@ def test_null_arguments_fail(fullchem, torch_dev, box):
                                  None, None) == -1
     assert b"negative ncell" in L.gckpp_last_error()
     ctx.close()
+
+
+def test_step_option_error_same_in_both_solvers(fullchem, torch_dev, box):
+    """An option error in the fused step: every cell reports it (after the
+    retry fails the same way), ISTATUS is zero, Hnew = RSTATUS(3) = 0, and C
+    is only clipped -- the same in the plain and the persistent step."""
+    print("case: fullchem step with ICNTRL(4) < 0, plain and persistent", flush=True)
+    from gckpp_amd import hip
+    T = torch_dev
+    w = box["w"]
+    ic = box["ic"].copy()
+    ic[3] = -4
+    out = {}
+    for solver in (hip.SOLVER_PLAIN, hip.SOLVER_PERSISTENT):
+        ctx = hip.Context(fullchem, 1024)
+        ctx.set_solver(solver)
+        conc = _t(T, w.conc)
+        hnew = T.full((w.ncell,), 123.0, dtype=T.float64, device="cuda")
+        hnew, ist, ierr, ok = ctx.fullchem_step(_t(T, w.met), _t(T, w.photol), _t(T, w.het), conc,
+                                                _t(T, w.hstart), 1200.0, ic, box["rc"], hnew=hnew)
+        assert not ok
+        out[solver] = (conc.cpu().numpy(), hnew.cpu().numpy(), ist.cpu().numpy(), ierr.cpu().numpy())
+        ctx.close()
+    a, b = out[hip.SOLVER_PLAIN], out[hip.SOLVER_PERSISTENT]
+    assert (a[3] == -1).all() and (a[1] == 0.0).all() and (a[2] == 0).all()
+    assert np.array_equal(a[0], np.maximum(w.conc, 0.0))
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_oversized_context_fails_cleanly(fullchem, torch_dev):
+    """A context larger than the device fails with a message and leaves the
+    device usable for the next context."""
+    print("case: gckpp_init beyond HBM", flush=True)
+    from gckpp_amd import hip
+    with pytest.raises(hip.GckppError, match="cannot allocate"):
+        hip.Context(fullchem, 2_000_000_000)
+    ctx = hip.Context(fullchem, 64)
+    ctx.close()
