@@ -797,8 +797,12 @@ extern "C" int gckpp_batch_init(gckpp_ctx *fullchem, gckpp_ctx *hg, gckpp_ctx *c
                 hipGetErrorString(e));
   }
   b->cap_waves = (int64_t)(wb / (sizeof(double) * 64 * (size_t)b->slot_fields));
-  HIP_OK(hipMalloc(&b->counter, sizeof(unsigned int)));
-  HIP_OK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+  e = hipMalloc(&b->counter, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    gckpp_batch_finalize(b);  // releases what was allocated
+    return fail("gckpp_batch_init: %s", hipGetErrorString(e));
+  }
   *out = b;
   return 0;
 }
@@ -942,13 +946,12 @@ extern "C" int gckpp_mlp_init(int64_t max_batch, const int32_t jslot[12], const 
       hipMalloc(&m->act, na) != hipSuccess ||
       hipMalloc(&m->dO, sizeof(float) * (size_t)max_batch * MLP_O) != hipSuccess ||
       hipMalloc(&m->dA, sizeof(float) * (size_t)max_batch * MLP_H) != hipSuccess ||
-      hipMalloc(&m->lossp, sizeof(float) * (size_t)((max_batch + 255) / 256)) != hipSuccess) {
-    delete m;
+      hipMalloc(&m->lossp, sizeof(float) * (size_t)((max_batch + 255) / 256)) != hipSuccess ||
+      hipMemset(m->params, 0, np) != hipSuccess || hipMemset(m->adam_m, 0, np) != hipSuccess ||
+      hipMemset(m->adam_v, 0, np) != hipSuccess) {
+    gckpp_mlp_finalize(m);  // releases what was allocated (the rest is null)
     return fail("gckpp_mlp_init: device allocation failed");
   }
-  HIP_OK(hipMemset(m->params, 0, np));
-  HIP_OK(hipMemset(m->adam_m, 0, np));
-  HIP_OK(hipMemset(m->adam_v, 0, np));
   m->packed_ok = false;
   *out = m;
   return 0;
