@@ -23,7 +23,11 @@ using namespace gckpp;
 
 static thread_local char g_err[512];
 
+// Records the message of a failed call.  The HIP runtime's last-error state
+// is consumed too: a failure reported here (a failed allocation, say) must
+// not be read back later as the error of an unrelated kernel launch.
 static int fail(const char *fmt, ...) {
+  (void)hipGetLastError();
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof g_err, fmt, ap);

@@ -130,7 +130,27 @@ def mech_sizes(mech_id):
 
 
 def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    """Device address of a tensor (None: a null pointer).  The kernels index
+    every array themselves, so a host or strided tensor is refused here
+    rather than read out of bounds on the device."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise GckppError("expected a device tensor, got one on %s" % t.device)
+    if not t.is_contiguous():
+        raise GckppError("expected a contiguous tensor, got strides %s" % (t.stride(),))
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _want(t, name, shape, dtype):
+    """Checks a caller's array: shape (None entries: any) and dtype."""
+    if t is None:
+        return
+    ok = t.dim() == len(shape) and all(w is None or w == g for w, g in zip(shape, t.shape))
+    if not ok:
+        raise GckppError("%s: expected shape %s, got %s" % (name, tuple(shape), tuple(t.shape)))
+    if t.dtype != dtype:
+        raise GckppError("%s: expected %s, got %s" % (name, dtype, t.dtype))
 
 
 def _stream():
@@ -150,6 +170,8 @@ class Context:
         _check(self.L.gckpp_init(mech.mech_id, int(max_cells), ctypes.byref(h)), "gckpp_init")
         self.h = h
         self.max_cells = int(max_cells)
+        sz = mech_sizes(mech.mech_id)
+        self.nmet, self.nhet, self.nphotol = sz[5], sz[6], sz[7]
         self.set_tolerances(mech.atol(), mech.rtol())
         sc = mech.species_consts
         self._keep = [np.ascontiguousarray(sc[k], dtype=np.float64) for k in ("mw", "henry_k0", "henry_cr")]
@@ -203,6 +225,10 @@ class Context:
         modified), keeps the fastest; returns ms[variant][log2 waves_per_group]."""
         nv = self.L.gckpp_persistent_variants()
         assert nv == PERSIST_VARIANTS, "library and binding disagree on the variant count"
+        n = conc.shape[1]
+        _want(conc, "conc", (self.mech.nspec, n), self.torch.float64)
+        _want(rconst, "rconst", (self.mech.nreact, n), self.torch.float64)
+        _want(hstart, "hstart", (n,), self.torch.float64)
         ms = (ctypes.c_float * (3 * nv))()
         ic = np.ascontiguousarray(icntrl, dtype=np.int32)
         rc = np.ascontiguousarray(rcntrl, dtype=np.float64)
@@ -220,10 +246,23 @@ class Context:
                                            ctypes.c_void_p(r.ctypes.data)), "gckpp_set_tolerances")
 
     # --- Update_RCONST -------------------------------------------------------
-    def rates(self, met, photol, het, conc, out=None):
+    def _inputs(self, met, photol, het, conc):
+        """Checks the per-cell input records; returns ncell."""
+        T, m = self.torch, self.mech
         n = conc.shape[1]
+        _want(conc, "conc", (m.nspec, n), T.float64)
+        _want(met, "met", (self.nmet, n), T.float64)
+        _want(photol, "photol", (self.nphotol, n), T.float64)
+        _want(het, "het", (self.nhet, None), T.float64)
+        if het.shape[1] not in (1, n):
+            raise GckppError("het: expected 1 or %d columns, got %d" % (n, het.shape[1]))
+        return n
+
+    def rates(self, met, photol, het, conc, out=None):
+        n = self._inputs(met, photol, het, conc)
         if out is None:
             out = self.torch.empty((self.mech.nreact, n), dtype=self.torch.float64, device=conc.device)
+        _want(out, "rconst", (self.mech.nreact, n), self.torch.float64)
         hs = 0 if het.shape[1] == 1 else n
         _check(self.L.gckpp_rates_dev(self.h, n, _ptr(met), _ptr(photol), _ptr(het), hs,
                                       _ptr(conc), _ptr(out), _stream()), "gckpp_rates_dev")
@@ -236,6 +275,12 @@ class Context:
         T = self.torch
         n = conc.shape[1]
         dev = conc.device
+        _want(conc, "conc", (self.mech.nspec, n), T.float64)
+        _want(rconst, "rconst", (self.mech.nreact, n), T.float64)
+        _want(hstart, "hstart", (n,), T.float64)
+        _want(istatus, "istatus", (20, n), T.int32)
+        _want(rstatus, "rstatus", (20, n), T.float64)
+        _want(ierr, "ierr", (n,), T.int32)
         if istatus is None:
             istatus = T.empty((20, n), dtype=T.int32, device=dev)
         if rstatus is None:
@@ -257,8 +302,12 @@ class Context:
         """Do_FullChem per-cell work for all cells; conc updated in place.
         Returns (hnew, istatus, ierr, ok) with ok False if a cell failed twice."""
         T = self.torch
-        n = conc.shape[1]
+        n = self._inputs(met, photol, het, conc)
         dev = conc.device
+        _want(hstart, "hstart", (n,), T.float64)
+        _want(hnew, "hnew", (n,), T.float64)
+        _want(istatus, "istatus", (20, n), T.int32)
+        _want(ierr, "ierr", (n,), T.int32)
         if hnew is None:
             hnew = T.empty((n,), dtype=T.float64, device=dev)
         if istatus is None:
@@ -285,13 +334,20 @@ class Context:
         return tuple(float(x) for x in ms)
 
     # --- components ----------------------------------------------------------
+    def _comp_in(self, conc, rconst):
+        n = conc.shape[1]
+        _want(conc, "conc", (self.mech.nspec, n), self.torch.float64)
+        _want(rconst, "rconst", (self.mech.nreact, n), self.torch.float64)
+
     def fun(self, conc, rconst):
+        self._comp_in(conc, rconst)
         out = self.torch.empty((self.mech.nvar, conc.shape[1]), dtype=self.torch.float64, device=conc.device)
         _check(self.L.gckpp_fun_dev(self.h, conc.shape[1], _ptr(conc), _ptr(rconst), _ptr(out), _stream()),
                "gckpp_fun_dev")
         return out
 
     def jac(self, conc, rconst):
+        self._comp_in(conc, rconst)
         out = self.torch.empty((self.mech.lu_nonzero, conc.shape[1]), dtype=self.torch.float64,
                                device=conc.device)
         _check(self.L.gckpp_jac_dev(self.h, conc.shape[1], _ptr(conc), _ptr(rconst), _ptr(out), _stream()),
@@ -299,6 +355,7 @@ class Context:
         return out
 
     def decomp(self, jvs):
+        _want(jvs, "jvs", (self.mech.lu_nonzero, jvs.shape[1]), self.torch.float64)
         lu = jvs.clone()
         ier = self.torch.empty((jvs.shape[1],), dtype=self.torch.int32, device=jvs.device)
         _check(self.L.gckpp_decomp_dev(self.h, jvs.shape[1], _ptr(lu), _ptr(ier), _stream()),
@@ -306,6 +363,8 @@ class Context:
         return lu, ier
 
     def solve(self, lu, x):
+        _want(lu, "lu", (self.mech.lu_nonzero, x.shape[1]), self.torch.float64)
+        _want(x, "x", (self.mech.nvar, x.shape[1]), self.torch.float64)
         out = x.clone()
         _check(self.L.gckpp_solve_dev(self.h, x.shape[1], _ptr(lu), _ptr(out), _stream()),
                "gckpp_solve_dev")

@@ -234,3 +234,24 @@ def test_persistent_variant_count_matches_library():
     if not hip.lib_path().exists():
         pytest.skip("HIP library not built")
     assert hip.load().gckpp_persistent_variants() == hip.PERSIST_VARIANTS
+
+
+def test_binding_refuses_host_strided_and_misshaped_tensors():
+    """The binding checks every array it hands to the kernels: a host tensor
+    or a strided view would be read out of bounds on the device, a wrong
+    shape or dtype misread; each is refused with a message before any HIP
+    call."""
+    import torch
+    from gckpp_amd import hip
+    with pytest.raises(hip.GckppError, match="device tensor"):
+        hip._ptr(torch.zeros(4, dtype=torch.float64))
+    assert hip._ptr(None) is None
+    t = torch.zeros((3, 8), dtype=torch.float64)
+    hip._want(t, "conc", (3, 8), torch.float64)
+    hip._want(t, "het", (3, None), torch.float64)
+    with pytest.raises(hip.GckppError, match="conc: expected shape"):
+        hip._want(t, "conc", (3, 9), torch.float64)
+    with pytest.raises(hip.GckppError, match="expected torch.int32"):
+        hip._want(t, "ierr", (3, 8), torch.int32)
+    with pytest.raises(hip.GckppError, match="expected shape"):
+        hip._want(t[:, ::2], "x", (3, 8), torch.float64)
