@@ -66,6 +66,8 @@ struct gckpp_ctx {
   int64_t pwaves;      // waves of one persistent launch (multiple of pwpb)
   // scratch for the host-pointer and retry paths
   std::vector<double> h_atol, h_rtol;
+  double *hs_d;        // host-pointer calls: device staging of one chunk (allocated on use)
+  int32_t *hs_i;
 };
 
 extern "C" const char *gckpp_last_error(void) { return g_err; }
@@ -170,7 +172,8 @@ extern "C" int gckpp_finalize(gckpp_ctx *c) {
   if (!c) return 0;
   bool ok = true;
   for (void *p : {(void *)c->work, (void *)c->atol, (void *)c->rtol, (void *)c->consts,
-                  (void *)c->step_buf, (void *)c->nfail, (void *)c->counter})
+                  (void *)c->step_buf, (void *)c->nfail, (void *)c->counter, (void *)c->hs_d,
+                  (void *)c->hs_i})
     if (p) ok = hipFree(p) == hipSuccess && ok;
   for (int k = 0; k < 5; ++k)
     if (c->ev[k]) ok = hipEventDestroy(c->ev[k]) == hipSuccess && ok;
@@ -527,12 +530,14 @@ __global__ __launch_bounds__(256) void k_sum_partials(int64_t nb, const int32_t 
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
-                                       const int32_t icntrl[20], const double rcntrl[20],
-                                       const double *met, const double *photol,
-                                       const double *het, int64_t het_stride,
-                                       const double *hstart, double *conc, double *hnew,
-                                       int32_t *istatus, int32_t *ierr, void *stream) {
+// The step of gckpp_fullchem_step_dev; *nfail receives the number of cells
+// that failed twice (not an error here).
+static int step_impl(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntrl[20],
+                     const double rcntrl[20], const double *met, const double *photol,
+                     const double *het, int64_t het_stride, const double *hstart, double *conc,
+                     double *hnew, int32_t *istatus, int32_t *ierr, void *stream,
+                     int32_t *nfail_out) {
+  *nfail_out = 0;
   if (!c) return fail("gckpp_fullchem_step: null context");
   if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_fullchem_step: negative ncell");
   if (ncell > c->wstride)
@@ -590,10 +595,8 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, partial, c->nfail);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(c->ev[4], s));
-    int32_t nfail = 0;
-    HIP_OK(hipMemcpyAsync(&nfail, c->nfail, sizeof nfail, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(nfail_out, c->nfail, sizeof *nfail_out, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    if (nfail > 0) return fail("gckpp_fullchem_step: %d cell(s) failed twice", nfail);
     return 0;
   }
   // C_before_integrate (:1026)
@@ -620,11 +623,159 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, partial, c->nfail);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(c->ev[4], s));
-  int32_t nfail = 0;
-  HIP_OK(hipMemcpyAsync(&nfail, c->nfail, sizeof nfail, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(nfail_out, c->nfail, sizeof *nfail_out, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
+                                       const int32_t icntrl[20], const double rcntrl[20],
+                                       const double *met, const double *photol,
+                                       const double *het, int64_t het_stride,
+                                       const double *hstart, double *conc, double *hnew,
+                                       int32_t *istatus, int32_t *ierr, void *stream) {
+  int32_t nfail = 0;
+  int rc = step_impl(c, ncell, dt, icntrl, rcntrl, met, photol, het, het_stride, hstart, conc,
+                     hnew, istatus, ierr, stream, &nfail);
+  if (rc) return rc;
   if (nfail > 0) return fail("gckpp_fullchem_step: %d cell(s) failed twice", nfail);
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host-pointer Update_RCONST and step: the caller's arrays stay in host
+// memory ([field][ncell], Fortran A(ncell, NFIELD)); the cells go through the
+// device in chunks of at most max_cells (a chunk of a [field][ncell] array is
+// a pitched 2-D block, one hipMemcpy2D per array), staged in buffers the
+// context allocates at the first such call.
+static int host_stage(gckpp_ctx *c, const char *who) {
+  if (c->hs_d) return 0;
+  const size_t n = (size_t)c->wstride;
+  const size_t nd = (size_t)(NMET + NPHOTOL + NHET + c->info.nspec + c->info.nreact + 2) * n;
+  const size_t ni = (size_t)21 * n;
+  hipError_t e = hipMalloc(&c->hs_d, sizeof(double) * nd);
+  if (e == hipSuccess) e = hipMalloc(&c->hs_i, sizeof(int32_t) * ni);
+  if (e != hipSuccess) {
+    if (c->hs_d) (void)hipFree(c->hs_d);
+    c->hs_d = nullptr;
+    c->hs_i = nullptr;
+    return fail("%s: cannot allocate %.2f GB of staging buffers: %s", who,
+                (sizeof(double) * nd + sizeof(int32_t) * ni) / 1e9, hipGetErrorString(e));
+  }
+  return 0;
+}
+// rows x n elements of a [rows][ncell] host array from cell c0, to or from a
+// [rows][n] device block.
+template <class T>
+static hipError_t stage_in(T *d, const T *h, int64_t rows, int64_t n, int64_t ncell, int64_t c0) {
+  return hipMemcpy2D(d, sizeof(T) * n, h + c0, sizeof(T) * ncell, sizeof(T) * n, rows,
+                     hipMemcpyHostToDevice);
+}
+template <class T>
+static hipError_t stage_out(T *h, const T *d, int64_t rows, int64_t n, int64_t ncell, int64_t c0) {
+  return hipMemcpy2D(h + c0, sizeof(T) * ncell, d, sizeof(T) * n, sizeof(T) * n, rows,
+                     hipMemcpyDeviceToHost);
+}
+struct Stage {
+  double *met, *photol, *het, *conc, *rconst, *hstart, *hnew;
+  int32_t *istatus, *ierr;
+};
+static Stage stage_views(const gckpp_ctx *c, int64_t n) {
+  Stage v;
+  v.met = c->hs_d;
+  v.photol = v.met + (size_t)NMET * n;
+  v.het = v.photol + (size_t)NPHOTOL * n;
+  v.conc = v.het + (size_t)NHET * n;
+  v.rconst = v.conc + (size_t)c->info.nspec * n;
+  v.hstart = v.rconst + (size_t)c->info.nreact * n;
+  v.hnew = v.hstart + n;
+  v.istatus = c->hs_i;
+  v.ierr = v.istatus + (size_t)20 * n;
+  return v;
+}
+// The per-cell inputs of one chunk.
+static hipError_t stage_inputs(const Stage &v, int64_t n, int64_t ncell, int64_t c0,
+                               const double *met, const double *photol, const double *het,
+                               int64_t het_stride, const double *conc, int nspec) {
+  hipError_t e = stage_in(v.met, met, NMET, n, ncell, c0);
+  if (e == hipSuccess) e = stage_in(v.photol, photol, NPHOTOL, n, ncell, c0);
+  if (e == hipSuccess)
+    e = het_stride ? stage_in(v.het, het, NHET, n, ncell, c0)
+                   : hipMemcpy(v.het, het, sizeof(double) * NHET, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = stage_in(v.conc, conc, nspec, n, ncell, c0);
+  return e;
+}
+
+extern "C" int gckpp_rates(gckpp_ctx *c, int64_t ncell, const double *met, const double *photol,
+                           const double *het, int64_t het_stride, const double *conc,
+                           double *rconst) {
+  if (!c) return fail("gckpp_rates: null context");
+  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_rates: negative ncell");
+  if (!met || !photol || !het || !conc || !rconst) return fail("gckpp_rates: null argument");
+  if (het_stride != 0 && het_stride != ncell)
+    return fail("gckpp_rates: het_stride must be 0 or ncell");
+  if (host_stage(c, "gckpp_rates")) return -1;
+  for (int64_t c0 = 0; c0 < ncell; c0 += c->wstride) {
+    const int64_t n = ncell - c0 < c->wstride ? ncell - c0 : c->wstride;
+    const Stage v = stage_views(c, n);
+    HIP_OK(stage_inputs(v, n, ncell, c0, met, photol, het, het_stride, conc, c->info.nspec));
+    int rc = gckpp_rates_dev(c, n, v.met, v.photol, v.het, het_stride ? n : 0, v.conc, v.rconst,
+                             nullptr);
+    if (rc) return rc;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(stage_out(rconst, v.rconst, c->info.nreact, n, ncell, c0));
+  }
+  return 0;
+}
+
+static int host_step_chunks(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntrl[20],
+                            const double rcntrl[20], const double *met, const double *photol,
+                            const double *het, int64_t het_stride, const double *hstart,
+                            double *conc, double *hnew, int32_t *istatus, int32_t *ierr) {
+  int64_t failed = 0;
+  for (int64_t c0 = 0; c0 < ncell; c0 += c->wstride) {
+    const int64_t n = ncell - c0 < c->wstride ? ncell - c0 : c->wstride;
+    const Stage v = stage_views(c, n);
+    HIP_OK(stage_inputs(v, n, ncell, c0, met, photol, het, het_stride, conc, c->info.nspec));
+    if (hstart) HIP_OK(stage_in(v.hstart, hstart, 1, n, ncell, c0));
+    else HIP_OK(hipMemset(v.hstart, 0, sizeof(double) * n));
+    int32_t nfail = 0;
+    int rc = step_impl(c, n, dt, icntrl, rcntrl, v.met, v.photol, v.het, het_stride ? n : 0,
+                       v.hstart, v.conc, v.hnew, v.istatus, v.ierr, nullptr, &nfail);
+    if (rc) return rc;
+    failed += nfail;
+    HIP_OK(stage_out(conc, v.conc, c->info.nspec, n, ncell, c0));
+    HIP_OK(stage_out(ierr, v.ierr, 1, n, ncell, c0));
+    if (hnew) HIP_OK(stage_out(hnew, v.hnew, 1, n, ncell, c0));
+    if (istatus) HIP_OK(stage_out(istatus, v.istatus, 20, n, ncell, c0));
+  }
+  if (failed > 0) return fail("gckpp_fullchem_step: %lld cell(s) failed twice", (long long)failed);
+  return 0;
+}
+
+extern "C" int gckpp_fullchem_step(gckpp_ctx *c, int64_t ncell, double dt,
+                                   const int32_t icntrl[20], const double rcntrl[20],
+                                   const double *met, const double *photol, const double *het,
+                                   int64_t het_stride, const double *hstart, double *conc,
+                                   double *hnew, int32_t *istatus, int32_t *ierr) {
+  if (!c) return fail("gckpp_fullchem_step: null context");
+  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_fullchem_step: negative ncell");
+  if (!met || !photol || !het || !conc || !ierr)
+    return fail("gckpp_fullchem_step: null argument");
+  if (het_stride != 0 && het_stride != ncell)
+    return fail("gckpp_fullchem_step: het_stride must be 0 or ncell");
+  if (host_stage(c, "gckpp_fullchem_step")) return -1;
+  // a cell order or step log set for device calls indexes their batch, not
+  // a chunk of this one: neither applies here
+  const int32_t *order = c->order;
+  float *hlog = c->hlog;
+  c->order = nullptr;
+  c->hlog = nullptr;
+  const int rc = host_step_chunks(c, ncell, dt, icntrl, rcntrl, met, photol, het, het_stride,
+                                  hstart, conc, hnew, istatus, ierr);
+  c->order = order;
+  c->hlog = hlog;
+  return rc;
 }
 
 // Device time of the parts of the last gckpp_fullchem_step_dev call, from HIP

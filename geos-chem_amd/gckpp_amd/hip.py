@@ -17,8 +17,9 @@ _lib = None
 
 SYMBOLS = [
     "gckpp_mech_sizes", "gckpp_init", "gckpp_finalize", "gckpp_set_tolerances",
-    "gckpp_set_species_consts", "gckpp_rates_dev", "gckpp_integrate_dev",
-    "gckpp_integrate", "gckpp_fullchem_step_dev", "gckpp_fun_dev", "gckpp_jac_dev",
+    "gckpp_set_species_consts", "gckpp_rates_dev", "gckpp_rates", "gckpp_integrate_dev",
+    "gckpp_integrate", "gckpp_fullchem_step_dev", "gckpp_fullchem_step", "gckpp_fun_dev",
+    "gckpp_jac_dev",
     "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_step_timings", "gckpp_set_solver",
     "gckpp_set_persistent_config", "gckpp_get_persistent_config", "gckpp_autotune_dev",
     "gckpp_persistent_variants",
@@ -71,6 +72,9 @@ def load():
     L.gckpp_set_tolerances.argtypes = [vp, dp, dp]
     L.gckpp_set_species_consts.argtypes = [vp, dp, dp, dp]
     L.gckpp_rates_dev.argtypes = [vp, i64, dp, dp, dp, i64, dp, dp, vp]
+    L.gckpp_rates.argtypes = [vp, i64, dp, dp, dp, i64, dp, dp]
+    L.gckpp_fullchem_step.argtypes = [vp, i64, ctypes.c_double, i32p, dp, dp, dp, dp, i64, dp, dp,
+                                      dp, i32p, i32p]
     L.gckpp_integrate_dev.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double, i32p, dp,
                                       dp, dp, dp, i32p, dp, i32p, vp]
     L.gckpp_integrate.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double, i32p, dp,
@@ -326,6 +330,40 @@ class Context:
             if "failed twice" not in msg:
                 raise GckppError("gckpp_fullchem_step_dev: " + msg)
         return hnew, istatus, ierr, rcode == 0
+
+    def fullchem_step_host(self, met, photol, het, conc, hstart, dt, icntrl, rcntrl):
+        """gckpp_fullchem_step on host (numpy) arrays, the Fortran host's call:
+        conc [NSPEC][n] float64 updated in place; returns (hnew, istatus,
+        ierr, ok) as numpy arrays."""
+        m = self.mech
+        n = conc.shape[1]
+        arrs = {"met": (met, (self.nmet, n)), "photol": (photol, (self.nphotol, n)),
+                "conc": (conc, (m.nspec, n))}
+        for name, (a, shp) in arrs.items():
+            if a.dtype != np.float64 or a.shape != shp or not a.flags.c_contiguous:
+                raise GckppError("%s: expected a C-contiguous float64 array of shape %s" % (name, shp))
+        if het.dtype != np.float64 or het.shape[0] != self.nhet or het.shape[1] not in (1, n) \
+                or not het.flags.c_contiguous:
+            raise GckppError("het: expected a C-contiguous float64 array [%d][1 or %d]" % (self.nhet, n))
+        hs = None
+        if hstart is not None:
+            hs = np.ascontiguousarray(hstart, dtype=np.float64)
+            if hs.shape != (n,):
+                raise GckppError("hstart: expected shape (%d,)" % n)
+        hnew = np.zeros(n)
+        ist = np.zeros((20, n), dtype=np.int32)
+        ierr = np.zeros(n, dtype=np.int32)
+        ic = np.ascontiguousarray(icntrl, dtype=np.int32)
+        rc = np.ascontiguousarray(rcntrl, dtype=np.float64)
+        h = lambda a: None if a is None else ctypes.c_void_p(a.ctypes.data)
+        rcode = self.L.gckpp_fullchem_step(self.h, n, float(dt), h(ic), h(rc), h(met), h(photol),
+                                           h(het), 0 if het.shape[1] == 1 else n, h(hs), h(conc),
+                                           h(hnew), h(ist), h(ierr))
+        if rcode != 0:
+            msg = self.L.gckpp_last_error().decode()
+            if "failed twice" not in msg:
+                raise GckppError("gckpp_fullchem_step: " + msg)
+        return hnew, ist, ierr, rcode == 0
 
     def step_timings(self):
         """(rates_ms, integrate_ms, retry_ms, total_ms) of the last fullchem_step."""

@@ -73,6 +73,11 @@ int gckpp_set_species_consts(gckpp_ctx *ctx, const double *mw,
 int gckpp_rates_dev(gckpp_ctx *ctx, int64_t ncell, const double *met,
                     const double *photol, const double *het, int64_t het_stride,
                     const double *conc, double *rconst, void *stream);
+/* The same with host arrays, synchronous; any ncell (the cells go through
+ * the device in chunks of at most max_cells, staged in buffers the context
+ * allocates at its first host-pointer call). */
+int gckpp_rates(gckpp_ctx *ctx, int64_t ncell, const double *met, const double *photol,
+                const double *het, int64_t het_stride, const double *conc, double *rconst);
 
 /* Integrate(TIN,TOUT,ICNTRL_U,RCNTRL_U,ISTATUS_U,RSTATUS_U,IERR_U) for ncell
  * cells with frozen rates (ICNTRL(15)=-1).  icntrl/rcntrl: the 20 options
@@ -101,6 +106,17 @@ int gckpp_fullchem_step_dev(gckpp_ctx *ctx, int64_t ncell, double dt,
                             const double *het, int64_t het_stride,
                             const double *hstart, double *conc, double *hnew,
                             int32_t *istatus, int32_t *ierr, void *stream);
+/* The same with host arrays, synchronous -- the one call of the host's
+ * Do_FullChem (the Fortran arrays C(ncell,NSPEC), MET(ncell,NMET), ...
+ * passed as they are); any ncell, in chunks of at most max_cells.  hstart
+ * may be null (cold start), hnew and istatus too (not returned).  The cell
+ * order and step log of gckpp_set_cell_order / gckpp_set_step_log do not
+ * apply to this call. */
+int gckpp_fullchem_step(gckpp_ctx *ctx, int64_t ncell, double dt,
+                        const int32_t icntrl[20], const double rcntrl[20],
+                        const double *met, const double *photol, const double *het,
+                        int64_t het_stride, const double *hstart, double *conc,
+                        double *hnew, int32_t *istatus, int32_t *ierr);
 
 /* Components, for parity checks against the reference routines:
  * Fun_SPLIT (gckpp_Function.F90:2172-5060): vdot[NVAR][ncell]

@@ -11,7 +11,7 @@ MODULE gckpp_hip_mod
   INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_SOLVER_PLAIN = 0, GCKPP_SOLVER_PERSISTENT = 1
   PUBLIC :: gckpp_init, gckpp_finalize, gckpp_set_tolerances, gckpp_set_species_consts
   PUBLIC :: gckpp_set_solver, gckpp_fullchem_step_dev, gckpp_integrate, gckpp_last_error
-  PUBLIC :: gckpp_mech_sizes
+  PUBLIC :: gckpp_mech_sizes, gckpp_rates, gckpp_fullchem_step
 
   INTERFACE
     ! out = NSPEC, NVAR, NFIX, NREACT, LU_NONZERO, NMET, NHET, NPHOTOL (no GPU needed)
@@ -60,6 +60,35 @@ MODULE gckpp_hip_mod
       INTEGER(C_INT32_T), INTENT(IN) :: icntrl(20)
       REAL(C_DOUBLE),     INTENT(IN) :: rcntrl(20)
       TYPE(C_PTR),        VALUE      :: met, photol, het, hstart, conc, hnew, istatus, ierr, stream
+    END FUNCTION
+    ! Update_RCONST with host arrays (synchronous): MET(ncell,NMET),
+    ! PHOTOL(ncell,NPHOTOL), HET(ncell or 1,NHET) (het_stride ncell or 0),
+    ! C(ncell,NSPEC) -> RCONST(ncell,NREACT)
+    INTEGER(C_INT) FUNCTION gckpp_rates(ctx, ncell, met, photol, het, het_stride, conc, rconst) &
+         BIND(C, name="gckpp_rates")
+      IMPORT :: C_INT, C_INT64_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR),        VALUE       :: ctx
+      INTEGER(C_INT64_T), VALUE       :: ncell, het_stride
+      REAL(C_DOUBLE),     INTENT(IN)  :: met(*), photol(*), het(*), conc(*)
+      REAL(C_DOUBLE),     INTENT(OUT) :: rconst(*)
+    END FUNCTION
+    ! One chemistry step with host arrays (synchronous): Do_FullChem's batch
+    ! call.  C(ncell,NSPEC) in/out, HSTART(ncell) = KPPHvalue in, HNEW(ncell)
+    ! the next KPPHvalue, ISTATUS(ncell,20), IERR(ncell); any ncell (chunks of
+    ! at most max_cells); returns -1 with "failed twice" when some cell's
+    ! retry failed (its IERR < 0)
+    INTEGER(C_INT) FUNCTION gckpp_fullchem_step(ctx, ncell, dt, icntrl, rcntrl, met, photol, het, &
+         het_stride, hstart, conc, hnew, istatus, ierr) BIND(C, name="gckpp_fullchem_step")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR),        VALUE         :: ctx
+      INTEGER(C_INT64_T), VALUE         :: ncell, het_stride
+      REAL(C_DOUBLE),     VALUE         :: dt
+      INTEGER(C_INT32_T), INTENT(IN)    :: icntrl(20)
+      REAL(C_DOUBLE),     INTENT(IN)    :: rcntrl(20)
+      REAL(C_DOUBLE),     INTENT(IN)    :: met(*), photol(*), het(*), hstart(*)
+      REAL(C_DOUBLE),     INTENT(INOUT) :: conc(*)
+      REAL(C_DOUBLE),     INTENT(OUT)   :: hnew(*)
+      INTEGER(C_INT32_T), INTENT(OUT)   :: istatus(*), ierr(*)
     END FUNCTION
     ! Host-pointer Integrate (synchronous): C(ncell,NSPEC), RCONST(ncell,NREACT), ...
     INTEGER(C_INT) FUNCTION gckpp_integrate(ctx, ncell, tin, tout, icntrl, rcntrl, hstart, conc, &
