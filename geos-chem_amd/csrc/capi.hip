@@ -68,6 +68,14 @@ struct gckpp_ctx {
   std::vector<double> h_atol, h_rtol;
   double *hs_d;        // host-pointer calls: device staging of one chunk (allocated on use)
   int32_t *hs_i;
+  // split persistent solve (gckpp_set_persistent_split): the first `split`
+  // cells of the order on the cell-per-wave kernel, on stream s2
+  int64_t split;
+  double *wwork;       // its slot blocks, wwaves waves
+  int64_t wwaves;
+  unsigned int *wcounter;
+  hipStream_t s2;
+  hipEvent_t ev_fork, ev_join;
 };
 
 extern "C" const char *gckpp_last_error(void) { return g_err; }
@@ -173,8 +181,11 @@ extern "C" int gckpp_finalize(gckpp_ctx *c) {
   bool ok = true;
   for (void *p : {(void *)c->work, (void *)c->atol, (void *)c->rtol, (void *)c->consts,
                   (void *)c->step_buf, (void *)c->nfail, (void *)c->counter, (void *)c->hs_d,
-                  (void *)c->hs_i})
+                  (void *)c->hs_i, (void *)c->wwork, (void *)c->wcounter})
     if (p) ok = hipFree(p) == hipSuccess && ok;
+  if (c->s2) ok = hipStreamDestroy(c->s2) == hipSuccess && ok;
+  if (c->ev_fork) ok = hipEventDestroy(c->ev_fork) == hipSuccess && ok;
+  if (c->ev_join) ok = hipEventDestroy(c->ev_join) == hipSuccess && ok;
   for (int k = 0; k < 5; ++k)
     if (c->ev[k]) ok = hipEventDestroy(c->ev[k]) == hipSuccess && ok;
   delete c;
@@ -298,6 +309,40 @@ static void fill_persist_queue(const gckpp_ctx *c, IntegrateArgs &a, int64_t nce
   a.clip_out = c->mech != MECH_CARBON;
 }
 
+// The persistent solve of one queue (pa.q[0], ncell cells): one launch of
+// the configured variant, or -- with a split set, a cell order and more
+// cells than the split -- the first `split` cells of the order on the
+// cell-per-wave kernel on the second stream, concurrently with the
+// configured variant on the rest of the order; work later enqueued on s
+// waits for both.
+static int launch_persist_queue(gckpp_ctx *c, PersistArgs &pa, int64_t ncell, hipStream_t s,
+                                const char *who) {
+  const bool can_split = c->split > 0 && c->wwork && pa.q[0].order && ncell > c->split &&
+                         c->mech != MECH_CARBON && c->pvariant != PERSIST_WAVE_VARIANT;
+  const int64_t k = can_split ? c->split : 0;
+  if (k > 0) {
+    PersistArgs pw = pa;
+    pw.qend[0] = pw.qend[1] = pw.qend[2] = k;
+    pw.work = c->wwork;
+    pw.counter = c->wcounter;
+    HIP_OK(hipEventRecord(c->ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+    if (launch_persist(c->mech, PERSIST_WAVE_VARIANT, 1, pw, k < c->wwaves ? k : c->wwaves,
+                       c->s2) != 0)
+      return fail("%s: persistent launch failed: %s", who, hipGetErrorString(hipGetLastError()));
+    HIP_OK(hipEventRecord(c->ev_join, c->s2));
+    pa.q[0].order += k;
+    pa.qend[0] = pa.qend[1] = pa.qend[2] = ncell - k;
+  }
+  int64_t nw = cells_to_waves(c->pvariant, ncell - k);
+  nw = (nw + c->pwpb - 1) / c->pwpb * c->pwpb;
+  if (nw > c->pwaves) nw = c->pwaves;
+  if (launch_persist(c->mech, c->pvariant, c->pwpb, pa, nw, s) != 0)
+    return fail("%s: persistent launch failed: %s", who, hipGetErrorString(hipGetLastError()));
+  if (k > 0) HIP_OK(hipStreamWaitEvent(s, c->ev_join, 0));
+  return 0;
+}
+
 static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
                           const int32_t icntrl[20], const double rcntrl[20],
                           const double *hstart, double *conc, const double *rconst,
@@ -344,12 +389,7 @@ static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
     pa.work = c->work;
     pa.slot_fields = persist_slot_fields(c->mech);
     pa.counter = c->counter;
-    int64_t nw = cells_to_waves(c->pvariant, ncell);
-    nw = (nw + c->pwpb - 1) / c->pwpb * c->pwpb;
-    if (nw > c->pwaves) nw = c->pwaves;
-    if (launch_persist(c->mech, c->pvariant, c->pwpb, pa, nw, s) != 0)
-      return fail("gckpp_integrate: persistent launch failed: %s", hipGetErrorString(hipGetLastError()));
-    return 0;
+    return launch_persist_queue(c, pa, ncell, s, "gckpp_integrate");
   }
   // chunks of at most wstride cells; outputs are indexed with the full ncell
   for (int64_t c0 = 0; c0 < ncell; c0 += c->wstride) {
@@ -577,12 +617,8 @@ static int step_impl(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntr
       pa.work = c->work;
       pa.slot_fields = persist_slot_fields(c->mech);
       pa.counter = c->counter;
-      int64_t nw = cells_to_waves(c->pvariant, ncell);
-      nw = (nw + c->pwpb - 1) / c->pwpb * c->pwpb;
-      if (nw > c->pwaves) nw = c->pwaves;
-      if (launch_persist(c->mech, c->pvariant, c->pwpb, pa, nw, s) != 0)
-        return fail("gckpp_fullchem_step: persistent launch failed: %s",
-                    hipGetErrorString(hipGetLastError()));
+      int rcl = launch_persist_queue(c, pa, ncell, s, "gckpp_fullchem_step");
+      if (rcl) return rcl;
     }
     HIP_OK(hipEventRecord(c->ev[3], s));
     int64_t nb = (ncell + 255) / 256;
@@ -1234,5 +1270,51 @@ extern "C" int gckpp_order_by_class_dev(int64_t ncell, const int32_t *cls, int32
   if (!cls || !order || !scratch) return fail("gckpp_order_by_class: null argument");
   order_by_class(ncell, cls, order, scratch, (hipStream_t)stream);
   HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gckpp_order_by_h0_dev(int64_t ncell, const double *h0, int32_t *order,
+                                     int32_t *scratch, void *stream) {
+  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_order_by_h0: negative ncell");
+  if (!h0 || !order || !scratch) return fail("gckpp_order_by_h0: null argument");
+  order_by_h0(ncell, h0, order, scratch, (hipStream_t)stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int64_t gckpp_order_by_h0_scratch(int64_t ncell) {
+  return ncell > 0 ? order_by_h0_scratch(ncell) : 0;
+}
+
+extern "C" int gckpp_set_persistent_split(gckpp_ctx *c, int64_t nfirst) {
+  if (!c) return fail("gckpp_set_persistent_split: null context");
+  if (nfirst < 0) return fail("gckpp_set_persistent_split: negative cell count");
+  if (nfirst > 0 && c->mech == MECH_CARBON)
+    return fail("gckpp_set_persistent_split: the carbon mechanism has no Rosenbrock solve");
+  if (nfirst > 0 && !c->wwork) {
+    const int64_t res = persist_resident_waves(c->mech, PERSIST_WAVE_VARIANT, 1);
+    if (res <= 0) return fail("gckpp_set_persistent_split: cannot size the cell-per-wave launch");
+    const size_t wb = sizeof(double) * (size_t)res * 64 * (size_t)persist_slot_fields(c->mech);
+    hipError_t e = hipMalloc(&c->wwork, wb);
+    if (e == hipSuccess) e = hipMalloc(&c->wcounter, sizeof(unsigned int));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      if (c->wwork) (void)hipFree(c->wwork);
+      if (c->wcounter) (void)hipFree(c->wcounter);
+      if (c->s2) (void)hipStreamDestroy(c->s2);
+      if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+      if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+      c->wwork = nullptr;
+      c->wcounter = nullptr;
+      c->s2 = nullptr;
+      c->ev_fork = c->ev_join = nullptr;
+      return fail("gckpp_set_persistent_split: %s (%.2f GB of slots)", hipGetErrorString(e),
+                  wb / 1e9);
+    }
+    c->wwaves = res;
+  }
+  c->split = nfirst;
   return 0;
 }

@@ -51,6 +51,10 @@ void mlp_grad_f32(const float *P, int64_t nb, const float *X, const float *yh, c
                   hipStream_t s);
 // order[0..n) = cells sorted by class, class 3 first, stable; scratch: 4*ceil(n/256) ints
 void order_by_class(int64_t n, const int32_t *cls, int32_t *order, int32_t *scratch, hipStream_t s);
+// order[0..n) = cells sorted by predicted h0, smallest first (32 bins of
+// log10 h0), stable; scratch: order_by_h0_scratch(n) ints
+void order_by_h0(int64_t n, const double *h0, int32_t *order, int32_t *scratch, hipStream_t s);
+int64_t order_by_h0_scratch(int64_t n);
 void mlp_sum(const float *x, int64_t n, float *out, hipStream_t s);  // out[0] = sum x[0..n)
 void mlp_adam(float *P, const float *G, float *m, float *v, float lr, int step, hipStream_t s);
 

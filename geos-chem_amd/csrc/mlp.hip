@@ -445,4 +445,87 @@ void order_by_class(int64_t n, const int32_t *cls, int32_t *order, int32_t *scra
   hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nb), dim3(256), 0, s, n, cls, scratch, order);
 }
 
+// ---------------------------------------------------------------------------
+// Stable counting sort of cells by predicted initial step size, smallest
+// (the stiffest, costliest cells) first: key = one of H0_BINS bins of
+// log10(h0) over [1e-5 s, 1200 s] (a quarter decade each); h0 <= 0 or not
+// finite (no prediction) sorts last.  Same three passes as the class sort,
+// over the keys stored in the scratch.
+constexpr int H0_BINS = 32;
+__global__ __launch_bounds__(256) void k_h0_key(int64_t n, const double *h0, int32_t *key) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double h = h0[i];
+  int k = H0_BINS - 1;
+  if (h > 0.0 && h < INFINITY) {
+    const double x = (log10(h) + 5.0) * (H0_BINS / (5.0 + log10(1200.0)));
+    k = x < 0.0 ? 0 : (x >= (double)(H0_BINS - 1) ? H0_BINS - 1 : (int)x);
+  }
+  key[i] = k;
+}
+__global__ __launch_bounds__(256) void k_key_hist(int64_t n, const int32_t *key, int32_t *hist) {
+  __shared__ int32_t cnt[H0_BINS];
+  if (threadIdx.x < H0_BINS) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[key[i]], 1);
+  __syncthreads();
+  if (threadIdx.x < H0_BINS) hist[(int64_t)blockIdx.x * H0_BINS + threadIdx.x] = cnt[threadIdx.x];
+}
+// In place: hist[b*K+k] <- start of block b's key-k cells (as k_class_scan).
+__global__ __launch_bounds__(256) void k_key_scan(int64_t nb, int32_t *hist) {
+  __shared__ int32_t tot[H0_BINS][256];
+  __shared__ int32_t base[H0_BINS];
+  const int64_t per = (nb + 255) / 256;
+  const int64_t b0 = (int64_t)threadIdx.x * per;
+  const int64_t b1 = b0 + per < nb ? b0 + per : nb;
+  for (int k = 0; k < H0_BINS; ++k) {
+    int32_t s = 0;
+    for (int64_t b = b0; b < b1; ++b) s += hist[b * H0_BINS + k];
+    tot[k][threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t run = 0;
+    for (int k = 0; k < H0_BINS; ++k) {
+      base[k] = run;
+      for (int t = 0; t < 256; ++t) run += tot[k][t];
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < H0_BINS; ++k) {
+    int32_t run = base[k];
+    for (int t = 0; t < (int)threadIdx.x; ++t) run += tot[k][t];
+    for (int64_t b = b0; b < b1; ++b) {
+      int32_t c = hist[b * H0_BINS + k];
+      hist[b * H0_BINS + k] = run;
+      run += c;
+    }
+  }
+}
+__global__ __launch_bounds__(256) void k_key_scatter(int64_t n, const int32_t *key,
+                                                     const int32_t *start, int32_t *order) {
+  __shared__ int32_t ks[256];
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int k = i < n ? key[i] : H0_BINS;
+  ks[threadIdx.x] = k;
+  __syncthreads();
+  if (i < n) {
+    int32_t r = 0;  // rank among the earlier cells of this block with the same key
+    for (int t = 0; t < (int)threadIdx.x; ++t) r += ks[t] == k;
+    order[start[(int64_t)blockIdx.x * H0_BINS + k] + r] = (int32_t)i;
+  }
+}
+
+int64_t order_by_h0_scratch(int64_t n) { return n + (int64_t)H0_BINS * ((n + 255) / 256); }
+
+void order_by_h0(int64_t n, const double *h0, int32_t *order, int32_t *scratch, hipStream_t s) {
+  int64_t nb = (n + 255) / 256;
+  int32_t *key = scratch, *hist = scratch + n;
+  hipLaunchKernelGGL(k_h0_key, dim3((unsigned)nb), dim3(256), 0, s, n, h0, key);
+  hipLaunchKernelGGL(k_key_hist, dim3((unsigned)nb), dim3(256), 0, s, n, key, hist);
+  hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(256), 0, s, nb, hist);
+  hipLaunchKernelGGL(k_key_scatter, dim3((unsigned)nb), dim3(256), 0, s, n, key, hist, order);
+}
+
 }  // namespace gckpp

@@ -29,7 +29,8 @@ SYMBOLS = [
     "gckpp_mlp_init", "gckpp_mlp_set_norm", "gckpp_mlp_set_params", "gckpp_mlp_get_params",
     "gckpp_mlp_features_dev", "gckpp_mlp_predict_dev", "gckpp_mlp_forward_dev",
     "gckpp_mlp_grad_dev", "gckpp_mlp_train_step_dev", "gckpp_mlp_finalize",
-    "gckpp_order_by_class_dev",
+    "gckpp_order_by_class_dev", "gckpp_order_by_h0_dev", "gckpp_order_by_h0_scratch",
+    "gckpp_set_persistent_split",
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
@@ -100,6 +101,9 @@ def load():
     L.gckpp_batch_finalize.argtypes = [vp]
     L.gckpp_set_cell_order.argtypes = [vp, vp]
     L.gckpp_set_step_log.argtypes = [vp, vp, ctypes.c_int]
+    L.gckpp_set_persistent_split.argtypes = [vp, i64]
+    L.gckpp_order_by_h0_dev.argtypes = [i64, vp, vp, vp, vp]
+    L.gckpp_order_by_h0_scratch.argtypes = [i64]
     fp = ctypes.POINTER(ctypes.c_float)
     L.gckpp_mlp_init.argtypes = [i64, vp, vp, ctypes.POINTER(vp)]
     L.gckpp_mlp_set_norm.argtypes = [vp, fp, fp, ctypes.c_float, ctypes.c_float]
@@ -115,8 +119,9 @@ def load():
     L.gckpp_mlp_finalize.argtypes = [vp]
     L.gckpp_order_by_class_dev.argtypes = [i64, vp, vp, vp, vp]
     for f in SYMBOLS:
-        if f not in ("gckpp_last_error",):
+        if f not in ("gckpp_last_error", "gckpp_order_by_h0_scratch"):
             getattr(L, f).restype = ctypes.c_int
+    L.gckpp_order_by_h0_scratch.restype = ctypes.c_int64
     _lib = L
     return L
 
@@ -205,6 +210,12 @@ class Context:
         solver takes cells; the tensor must outlive its use."""
         self._order = order
         _check(self.L.gckpp_set_cell_order(self.h, _ptr(order)), "gckpp_set_cell_order")
+
+    def set_persistent_split(self, nfirst):
+        """The first nfirst cells of the cell order run on the cell-per-wave
+        kernel on a second stream, concurrently with the lane kernel on the
+        rest (0: off); see gckpp_set_persistent_split."""
+        _check(self.L.gckpp_set_persistent_split(self.h, int(nfirst)), "gckpp_set_persistent_split")
 
     def set_step_log(self, hlog, nattempt=0):
         """Device float tensor [2*nattempt][ncell] (or None) receiving (H, err)
@@ -472,6 +483,22 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def order_by_h0(h0):
+    """Cells sorted by predicted h0, smallest (stiffest) first, in quarter-
+    decade bins, stable: int32 device tensor."""
+    import torch
+    n = h0.shape[0]
+    if h0.dtype != torch.float64:
+        raise GckppError("h0: expected torch.float64, got %s" % h0.dtype)
+    L = load()
+    order = torch.empty((n,), dtype=torch.int32, device=h0.device)
+    scratch = torch.empty((max(1, int(L.gckpp_order_by_h0_scratch(n))),), dtype=torch.int32,
+                          device=h0.device)
+    _check(L.gckpp_order_by_h0_dev(n, _ptr(h0), _ptr(order), _ptr(scratch), _stream()),
+           "gckpp_order_by_h0_dev")
+    return order
 
 
 def order_by_class(cls):

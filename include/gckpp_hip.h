@@ -168,6 +168,19 @@ int gckpp_get_persistent_config(gckpp_ctx *ctx, int32_t out[7]);
 int gckpp_set_cell_order(gckpp_ctx *ctx, const int32_t *order);
 int gckpp_set_step_log(gckpp_ctx *ctx, float *hlog, int nattempt);
 
+/* Split persistent solve (no reference counterpart).  With nfirst > 0 and a
+ * cell order set, the next persistent Integrate or step runs the first
+ * nfirst cells of the order on the cell-per-wave kernel (variant 7) on a
+ * second stream of the context, concurrently with the configured variant on
+ * the rest of the order; the call's stream waits for both.  Meant for the
+ * costliest cells of an uneven step (a warm step, cells ordered by
+ * gckpp_order_by_h0_dev): one cell-per-wave attempt takes far less time than
+ * a lane-per-cell attempt, so the longest chains of attempts stop setting
+ * the step's length.  Results are bitwise those of one kernel.  0 turns it
+ * off; ignored without a cell order, for carbon and when nfirst >= ncell.
+ * The first call with nfirst > 0 allocates the cell-per-wave slots. */
+int gckpp_set_persistent_split(gckpp_ctx *ctx, int64_t nfirst);
+
 /* Autotuner: integrates a copy of the given cells with every persistent
  * configuration, keeps the fastest for this context and writes the times
  * [ms] to ms[variant*3 + log2(waves_per_group)] (-1: does not fit).  conc is
@@ -261,6 +274,13 @@ int gckpp_mlp_finalize(gckpp_mlp *m);
  * scratch: 4*ceil(ncell/256) int32 on the device. */
 int gckpp_order_by_class_dev(int64_t ncell, const int32_t *cls, int32_t *order,
                              int32_t *scratch, void *stream);
+/* Order of cells by predicted initial step size, smallest first (the
+ * stiffest cells), stable: a counting sort over 32 quarter-decade bins of
+ * log10(h0[ncell]) on [1e-5 s, 1200 s] (h0 <= 0: last).  scratch:
+ * gckpp_order_by_h0_scratch(ncell) int32 (device). */
+int gckpp_order_by_h0_dev(int64_t ncell, const double *h0, int32_t *order, int32_t *scratch,
+                          void *stream);
+int64_t gckpp_order_by_h0_scratch(int64_t ncell);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *gckpp_last_error(void);

@@ -327,3 +327,68 @@ def test_mixed_batch_bitwise_and_graph_replays(variant, fullchem, hg, carbon, to
     B.close()
     for c in ctxs:
         c.close()
+
+
+def test_order_by_h0_is_a_stable_bin_sort(torch_dev):
+    """gckpp_order_by_h0_dev: a permutation of the cells, bins of log10(h0)
+    (quarter decades from 1e-5 s) nondecreasing along it, increasing cell
+    index within a bin; h0 <= 0 last."""
+    print("case: order by predicted h0, 50,000 cells", flush=True)
+    from gckpp_amd import hip
+    T = torch_dev
+    rng = np.random.default_rng(5)
+    n = 50000
+    width = (5.0 + np.log10(1200.0)) / 32
+    b = rng.integers(0, 32, n)
+    h0 = 10.0 ** (-5.0 + (b + rng.uniform(0.2, 0.8, n)) * width)   # away from the bin edges
+    h0[rng.integers(0, n, 300)] = 0.0
+    order = hip.order_by_h0(_t(T, h0)).cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(n))
+    key = np.where(h0 > 0, np.clip(np.floor((np.log10(np.where(h0 > 0, h0, 1.0)) + 5.0) / width), 0, 31), 31)
+    k = key[order]
+    assert (np.diff(k) >= 0).all()
+    same = np.diff(k) == 0
+    assert (np.diff(order)[same] > 0).all()
+
+
+@pytest.mark.parametrize("name,forced", [("fullchem", False), ("fullchem", True), ("Hg", False)])
+def test_split_persistent_step_bitwise(name, forced, mech_of, torch_dev):
+    """The split persistent step -- the first cells of the h0 order on the
+    cell-per-wave kernel on a second stream, the rest on the lane kernel at
+    the same time -- gives bitwise the single-kernel step's results, also
+    when cells fail and take the retry."""
+    print("case: split persistent step, %s, forced retries %s" % (name, forced), flush=True)
+    from gckpp_amd import hip, synth
+    from oracle.ref import icntrl_for
+    T = torch_dev
+    m = mech_of(name)
+    w = synth.box_cells(m, 1024, 9)
+    ic, rc = icntrl_for(name)
+    hs = w.hstart
+    if forced:
+        ic = ic.copy()
+        ic[3] = 40
+        hs = np.full(w.ncell, 1e-13)
+    rng = np.random.default_rng(2)
+    h0_guess = _t(T, 10.0 ** rng.uniform(-5, 3, w.ncell))
+    order = hip.order_by_h0(h0_guess)
+    out = {}
+    for split in (0, 150):
+        ctx = hip.Context(m, 2048)
+        ctx.set_solver(hip.SOLVER_PERSISTENT)
+        ctx.set_cell_order(order)
+        ctx.set_persistent_split(split)
+        conc = _t(T, w.conc)
+        hnew, ist, ierr, ok = ctx.fullchem_step(_t(T, w.met), _t(T, w.photol), _t(T, w.het), conc,
+                                                _t(T, hs), 1200.0, ic, rc)
+        out[split] = (conc.cpu().numpy(), hnew.cpu().numpy(), ist.cpu().numpy(), ierr.cpu().numpy())
+        # Integrate alone, split the same way
+        c2 = _t(T, w.conc)
+        rct = ctx.rates(_t(T, w.met), _t(T, w.photol), _t(T, w.het), c2)
+        ist2, rst2, ierr2 = ctx.integrate(c2, rct, _t(T, hs), 0.0, 1200.0, ic, rc)
+        out[split] += (c2.cpu().numpy(), ist2.cpu().numpy(), ierr2.cpu().numpy())
+        ctx.close()
+    if forced:
+        assert (out[0][3] < 0).sum() > 20 and (out[0][3] == 1).sum() > 20
+    for x, y in zip(out[0], out[150]):
+        assert np.array_equal(x, y)
