@@ -56,9 +56,12 @@ def parse():
                    help="persistent kernel variant (default: autotuned on the workload)")
     p.add_argument("--wpb", type=int, default=1, help="waves per workgroup with --variant")
     p.add_argument("--no-warm", action="store_true", help="skip the warm-step legs")
-    p.add_argument("--split", type=int, default=16384,
+    p.add_argument("--split", type=int, default=4096,
                    help="warm step: cells (those of the smallest predicted h0) run on the "
                         "cell-per-wave kernel beside the lane kernel, per launch (0: one kernel)")
+    p.add_argument("--handoff", type=int, default=32768,
+                   help="persistent solver: once at most this many cells of a lane-kernel launch "
+                        "are unfinished, they move to the cell-per-wave kernel (0: off)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device copy (reference for HBM byte counters)")
     return p.parse_args()
@@ -186,7 +189,9 @@ def grid_bench(args):
         log("no trained model at %s: the learned leg runs with cold Hstart" % mpath)
 
     # autotune the persistent solver on the workload itself (a sample smaller
-    # than the resident lanes would time a different, latency-only regime)
+    # than the resident lanes would time a different, latency-only regime),
+    # with the tail hand-off it will run with
+    ctx.set_persistent_handoff(args.handoff)
     tune_ms = None
     if args.variant >= 0:
         ctx.set_persistent_config(args.variant, args.wpb)
@@ -225,26 +230,42 @@ def grid_bench(args):
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
-        tim = []
+        tim, hos = [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             fn()
             tim.append(ctx.step_timings())
+            if name == "learned" and args.handoff > 0:
+                hos.append(ctx.handoff_stats())
+                log("hand-off: %d cells, %d attempts; lane launch %.1f ms, cell-per-wave launch "
+                    "%.1f ms" % hos[-1])
             log("%s step done: %.1f ms total, integrate %.1f ms" % (name, tim[-1][3], tim[-1][1]))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         st = ist.cpu().numpy()
         legs[name] = dict(ms=wall / args.steps * 1e3, cps=n / (wall / args.steps), tim=tim,
                           conc=conc.cpu().numpy(), st=st, ierr=ierr.cpu().numpy(),
-                          hnew=hnew.clone())
+                          hnew=hnew.clone(), hos=hos)
     ctx.set_cell_order(None)
     L, P = legs["learned"], legs["plain"]
     warm = None if args.no_warm else warm_legs(args, mech, w, ctx, model, P, ic, rc)
 
-    # dominant kernel: the persistent integrate of the learned leg
+    # dominant kernel: the persistent integrate (k_persist) of the learned
+    # leg; with the hand-off, its own launch (HIP events around it) and its
+    # own attempts (all attempts less those of the cell-per-wave launch,
+    # counted at the first-attempt cost: rejections are 0.3 % of attempts)
     nacc, nstp = L["st"][3].astype(np.int64), L["st"][2].astype(np.int64)
     flop = FLOP_ACC * nacc.sum() + FLOP_REJ * (nstp - nacc).sum()
     k_ms = float(np.mean([t[1] for t in L["tim"]]))
+    handoff = None
+    if L["hos"]:
+        ho = np.array(L["hos"], dtype=np.float64).mean(axis=0)
+        flop -= FLOP_ACC * ho[1]
+        k_ms = float(ho[2])
+        handoff = {"max_cells": args.handoff, "cells_handed_off": int(round(ho[0])),
+                   "cell_per_wave_attempts": int(round(ho[1])), "lane_launch_ms": round(ho[2], 3),
+                   "cell_per_wave_launch_ms": round(ho[3], 3),
+                   "integrate_ms": round(float(np.mean([t[1] for t in L["tim"]])), 3)}
     achieved = flop / (k_ms * 1e-3) / 1e12
     # HBM bytes per launch from the committed rocprof pass (FETCH_SIZE and
     # WRITE_SIZE, copy-calibrated), when it measured this variant on this grid
@@ -252,7 +273,8 @@ def grid_bench(args):
     tfile = ROOT / "profiles" / "persist_traffic.json"
     if tfile.exists():
         for tr in json.loads(tfile.read_text()).get("entries", []):
-            if tr.get("variant") == pcfg["variant"] and tr.get("cells") == n:
+            if (tr.get("variant") == pcfg["variant"] and tr.get("cells") == n and
+                    tr.get("handoff", 0) == args.handoff):
                 traffic = tr.get("bytes_per_launch")
 
     base, sidx, refout = cpu_baseline(mech, w, ic, rc, args.cpu_sample)
@@ -291,7 +313,8 @@ def grid_bench(args):
                                "%d cells, dt=1200 s, cold start, repo RTOL/ATOL; learned h0 (bf16 "
                                "MFMA MLP) + cost-ordered persistent Rodas3" % (nx, ny, nz, n),
                    "cells": n, "learned_model": model is not None, "h0_safety": args.safety,
-                   "persistent_config": pcfg},
+                   "persistent_config": pcfg, "handoff_max_cells": args.handoff},
+        "handoff": handoff,
         "plain_gpu": {"value": round(P["cps"], 2), "unit": "cells/s", "ms_per_step": round(P["ms"], 3),
                       "solver": "one lane per cell (k_integrate), Hstart 1e-5 s"},
         "speedup_vs_plain": round(L["cps"] / P["cps"], 3),
@@ -530,6 +553,9 @@ def c90_bench(args):
     ic, rc = fullchem_icntrl()
     nface = 90 * 90 * 72
     ctx = hip.Context(mech, nface)
+    if args.variant >= 0:
+        ctx.set_persistent_config(args.variant, args.wpb)
+    ctx.set_persistent_handoff(args.handoff)
     mpath = learned.default_model_path("fullchem")
     model = learned.StepModel(mech).load(mpath) if mpath.exists() else None
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
@@ -550,6 +576,7 @@ def c90_bench(args):
     def run(use_learned, warm):
         """One step of every face; returns wall s, kernel ms, attempts, ..."""
         fails, att, acc_att, kern_ms = 0, [], 0, 0.0
+        ho = [0, 0, 0.0, 0.0]   # hand-off: cells, cell-per-wave attempts, lane ms, cell-per-wave ms
         ctx.set_solver(hip.SOLVER_PERSISTENT if use_learned else hip.SOLVER_PLAIN)
         outs = []
         wall = 0.0
@@ -575,6 +602,8 @@ def c90_bench(args):
             torch.cuda.synchronize()
             wall += time.perf_counter() - t0
             kern_ms += float(ctx.step_timings()[1])
+            if use_learned and args.handoff > 0:
+                ho = [a + b for a, b in zip(ho, ctx.handoff_stats())]
             fails += int((ierr < 0).sum().item())
             att.append(float(ist[2].float().mean().item()))
             acc_att += int(ist[3].sum().item())
@@ -584,7 +613,7 @@ def c90_bench(args):
         ctx.set_cell_order(None)
         log("c90 %s %s step: %.1f s" % ("learned + persistent" if use_learned else "plain",
                                          "warm" if warm else "cold", wall))
-        return wall, kern_ms, fails, att, outs
+        return wall, kern_ms, fails, att, outs, ho
 
     res = {}
     for warm in (False, True):
@@ -599,7 +628,7 @@ def c90_bench(args):
     n = 6 * nface
 
     def leg_json(key):
-        wall, kms, fails, att, outs = res[key]
+        wall, kms, fails, att, outs, ho = res[key]
         return {"value": round(n / wall, 2), "unit": "cells/s", "ms_per_step": round(wall * 1e3, 3),
                 "integrate_kernel_ms": round(kms, 3), "failed_cells": fails,
                 "mean_attempts_per_face": [round(a, 3) for a in att]}
@@ -638,6 +667,10 @@ def c90_bench(args):
     for (_, _, nacc, ntot) in res[("learned", False)][4]:
         flop += FLOP_ACC * nacc + FLOP_REJ * (ntot - nacc)
     kms = res[("learned", False)][1]
+    ho = res[("learned", False)][5]
+    if args.handoff > 0:   # k_persist's own launches and attempts (see grid_bench)
+        flop -= FLOP_ACC * ho[1]
+        kms = ho[2]
     achieved = flop / (kms * 1e-3) / 1e12
     L, P = leg_json(("learned", False)), leg_json(("plain", False))
     Lw, Pw = leg_json(("learned", True)), leg_json(("plain", True))
@@ -649,7 +682,11 @@ def c90_bench(args):
                                    "the warm step after a synthetic emission/transport perturbation; "
                                    "learned h0 + persistent against plain GPU",
                        "learned_model": model is not None,
-                       "persistent_config": ctx.persistent_config()},
+                       "persistent_config": ctx.persistent_config(),
+                       "handoff_max_cells": args.handoff},
+            "handoff_cold": None if args.handoff <= 0 else {
+                "cells_handed_off": ho[0], "cell_per_wave_attempts": ho[1],
+                "lane_launch_ms": round(ho[2], 3), "cell_per_wave_launch_ms": round(ho[3], 3)},
             "learned": L, "plain_gpu": P, "speedup_vs_plain": round(P["ms_per_step"] / L["ms_per_step"], 3),
             "warm_step": {"learned": Lw, "plain_kpphvalue": Pw,
                           "speedup_vs_plain": round(Pw["ms_per_step"] / Lw["ms_per_step"], 3),

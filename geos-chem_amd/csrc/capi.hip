@@ -76,6 +76,17 @@ struct gckpp_ctx {
   unsigned int *wcounter;
   hipStream_t s2;
   hipEvent_t ev_fork, ev_join;
+  // tail hand-off (gckpp_set_persistent_handoff): at most ho_cap cells of a
+  // lane-kernel launch move to a cell-per-wave launch after it
+  int64_t ho_cap, ho_alloc;
+  unsigned int *ho_ctr;  // [4]: cells done, cells handed off, the resume launch's counter,
+                         // the resume launch's attempts
+  hipEvent_t ev_ho[3];   // lane launch start | resume start | resume end
+  int32_t *ho_cell;      // [ho_alloc]
+  double *ho_rec;        // [HO_FIELDS + NVAR][ho_cap] (room for ho_alloc)
+  double *hwork;         // scratch of the resume launch, hwaves waves of hw_fields*64
+  int64_t hwaves, hw_fields;
+  bool ho_timed;         // ev_ho recorded by a launch
 };
 
 extern "C" const char *gckpp_last_error(void) { return g_err; }
@@ -181,11 +192,14 @@ extern "C" int gckpp_finalize(gckpp_ctx *c) {
   bool ok = true;
   for (void *p : {(void *)c->work, (void *)c->atol, (void *)c->rtol, (void *)c->consts,
                   (void *)c->step_buf, (void *)c->nfail, (void *)c->counter, (void *)c->hs_d,
-                  (void *)c->hs_i, (void *)c->wwork, (void *)c->wcounter})
+                  (void *)c->hs_i, (void *)c->wwork, (void *)c->wcounter, (void *)c->ho_ctr,
+                  (void *)c->ho_cell, (void *)c->ho_rec, (void *)c->hwork})
     if (p) ok = hipFree(p) == hipSuccess && ok;
   if (c->s2) ok = hipStreamDestroy(c->s2) == hipSuccess && ok;
   if (c->ev_fork) ok = hipEventDestroy(c->ev_fork) == hipSuccess && ok;
   if (c->ev_join) ok = hipEventDestroy(c->ev_join) == hipSuccess && ok;
+  for (int k = 0; k < 3; ++k)
+    if (c->ev_ho[k]) ok = hipEventDestroy(c->ev_ho[k]) == hipSuccess && ok;
   for (int k = 0; k < 5; ++k)
     if (c->ev[k]) ok = hipEventDestroy(c->ev[k]) == hipSuccess && ok;
   delete c;
@@ -314,7 +328,9 @@ static void fill_persist_queue(const gckpp_ctx *c, IntegrateArgs &a, int64_t nce
 // cells than the split -- the first `split` cells of the order on the
 // cell-per-wave kernel on the second stream, concurrently with the
 // configured variant on the rest of the order; work later enqueued on s
-// waits for both.
+// waits for both.  With a hand-off set (lane variants), the lane kernel's
+// last unfinished cells move at a step boundary to a cell-per-wave launch
+// that follows it on s.
 static int launch_persist_queue(gckpp_ctx *c, PersistArgs &pa, int64_t ncell, hipStream_t s,
                                 const char *who) {
   const bool can_split = c->split > 0 && c->wwork && pa.q[0].order && ncell > c->split &&
@@ -334,11 +350,31 @@ static int launch_persist_queue(gckpp_ctx *c, PersistArgs &pa, int64_t ncell, hi
     pa.q[0].order += k;
     pa.qend[0] = pa.qend[1] = pa.qend[2] = ncell - k;
   }
+  const bool hand = c->ho_cap > 0 && c->ho_rec && c->mech != MECH_CARBON &&
+                    c->pvariant != PERSIST_WAVE_VARIANT && ncell - k > 0;
+  if (hand) {
+    launch_zero_u32(c->ho_ctr, 4, s);
+    pa.ho = Handoff{c->ho_ctr, c->ho_ctr + 1, c->ho_ctr + 3, c->ho_cell, c->ho_rec, c->ho_cap, 0};
+    HIP_OK(hipEventRecord(c->ev_ho[0], s));
+  }
   int64_t nw = cells_to_waves(c->pvariant, ncell - k);
   nw = (nw + c->pwpb - 1) / c->pwpb * c->pwpb;
   if (nw > c->pwaves) nw = c->pwaves;
   if (launch_persist(c->mech, c->pvariant, c->pwpb, pa, nw, s) != 0)
     return fail("%s: persistent launch failed: %s", who, hipGetErrorString(hipGetLastError()));
+  if (hand) {
+    HIP_OK(hipEventRecord(c->ev_ho[1], s));
+    PersistArgs ph = pa;
+    ph.ho.resume = 1;
+    ph.work = c->hwork;
+    ph.slot_fields = c->hw_fields;
+    ph.counter = c->ho_ctr + 2;
+    if (launch_persist(c->mech, PERSIST_WAVE_VARIANT, 1, ph, c->hwaves, s) != 0)
+      return fail("%s: hand-off launch failed: %s", who, hipGetErrorString(hipGetLastError()));
+    HIP_OK(hipEventRecord(c->ev_ho[2], s));
+    c->ho_timed = true;
+    pa.ho = Handoff{};
+  }
   if (k > 0) HIP_OK(hipStreamWaitEvent(s, c->ev_join, 0));
   return 0;
 }
@@ -1316,5 +1352,63 @@ extern "C" int gckpp_set_persistent_split(gckpp_ctx *c, int64_t nfirst) {
     c->wwaves = res;
   }
   c->split = nfirst;
+  return 0;
+}
+
+extern "C" int gckpp_set_persistent_handoff(gckpp_ctx *c, int64_t max_cells) {
+  if (!c) return fail("gckpp_set_persistent_handoff: null context");
+  if (max_cells < 0) return fail("gckpp_set_persistent_handoff: negative cell count");
+  if (max_cells > 0 && c->mech == MECH_CARBON)
+    return fail("gckpp_set_persistent_handoff: the carbon mechanism has no Rosenbrock solve");
+  if (max_cells > INT32_MAX)
+    return fail("gckpp_set_persistent_handoff: %lld cells exceed the list's index range",
+                (long long)max_cells);
+  if (max_cells > c->ho_alloc) {
+    // (re)allocate the list for max_cells; the resume launch's scratch once
+    const MechInfo mi = mech_info(c->mech);
+    for (void *p : {(void *)c->ho_cell, (void *)c->ho_rec})
+      if (p) (void)hipFree(p);
+    c->ho_cell = nullptr;
+    c->ho_rec = nullptr;
+    c->ho_cap = c->ho_alloc = 0;
+    hipError_t e = hipMalloc(&c->ho_cell, sizeof(int32_t) * (size_t)max_cells);
+    if (e == hipSuccess)
+      e = hipMalloc(&c->ho_rec, sizeof(double) * (size_t)(HO_FIELDS + mi.nvar) * (size_t)max_cells);
+    if (e == hipSuccess && !c->ho_ctr) e = hipMalloc(&c->ho_ctr, 4 * sizeof(unsigned int));
+    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+      if (!c->ev_ho[k]) e = hipEventCreate(&c->ev_ho[k]);
+    if (e == hipSuccess && !c->hwork) {
+      const int64_t res = persist_resident_waves(c->mech, PERSIST_WAVE_VARIANT, 1);
+      if (res <= 0) e = hipErrorInvalidConfiguration;
+      c->hw_fields = (mi.nreact + 5 * (int64_t)mi.nvar + 63) / 64;
+      c->hwaves = res;
+      if (e == hipSuccess)
+        e = hipMalloc(&c->hwork, sizeof(double) * 64 * (size_t)c->hw_fields * (size_t)res);
+    }
+    if (e != hipSuccess) {
+      for (void *p : {(void *)c->ho_cell, (void *)c->ho_rec})
+        if (p) (void)hipFree(p);
+      c->ho_cell = nullptr;
+      c->ho_rec = nullptr;
+      return fail("gckpp_set_persistent_handoff: %s", hipGetErrorString(e));
+    }
+    c->ho_alloc = max_cells;
+  }
+  c->ho_cap = max_cells;
+  return 0;
+}
+
+extern "C" int gckpp_persistent_handoff_stats(gckpp_ctx *c, int64_t n[2], float ms[2]) {
+  if (!c || !n || !ms) return fail("gckpp_persistent_handoff_stats: null argument");
+  n[0] = n[1] = 0;
+  ms[0] = ms[1] = 0.0f;
+  if (!c->ho_ctr || !c->ho_timed) return 0;
+  unsigned int v[4];
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(v, c->ho_ctr, sizeof v, hipMemcpyDeviceToHost));
+  n[0] = (int64_t)v[1];
+  n[1] = (int64_t)v[3];
+  HIP_OK(hipEventElapsedTime(&ms[0], c->ev_ho[0], c->ev_ho[1]));
+  HIP_OK(hipEventElapsedTime(&ms[1], c->ev_ho[1], c->ev_ho[2]));
   return 0;
 }

@@ -53,6 +53,25 @@ struct IntegrateArgs {
   int hlog_n;
 };
 
+// Tail hand-off of a one-queue persistent launch (not the Euler mechanism).
+// The lane kernel, once at most `cap` of its cells are unfinished, stops each
+// cell at its next step boundary (before the first attempt of a step, where
+// the integrator state is T, H, the counters and Y) and saves that state here;
+// a cell-per-wave launch after it (resume = 1) continues those cells.  Every
+// cell handed off was one of the at most `cap` unfinished cells when the
+// threshold was first met, so the list never overflows (an entry past `cap`
+// is not used: that cell stays on its lane).
+struct Handoff {
+  unsigned int *done;    // lane kernel: cells finished or handed off
+  unsigned int *count;   // cells handed off (entries used: min(count, cap))
+  unsigned int *attempts;  // k_wave resume: attempts it made (a statistic)
+  int32_t *cell;         // [cap] the handed-off cells
+  double *rec;           // [HO_FIELDS + NVAR][cap] their state, field-major
+  int64_t cap;           // 0: no hand-off
+  int resume;            // k_wave: take the cells of the list and resume them
+};
+constexpr int HO_FIELDS = 14;  // T, H, Hexit, Texit, HnewAcc, ISTATUS(1:8), tries
+
 // One launch of the persistent solver: up to three queues of cells (one per
 // mechanism); global index c < qend[0] is cell c of queue 0, and so on.
 struct PersistArgs {
@@ -61,6 +80,7 @@ struct PersistArgs {
   double *work;          // [nblocks][slot_fields][64]
   int64_t slot_fields;
   unsigned int *counter; // work-stealing counter, or null (one cell per lane)
+  Handoff ho;
 };
 
 struct RatesArgs {
@@ -106,6 +126,7 @@ int64_t persist_slot_fields(int qset);                   // doubles per lane slo
 int64_t persist_resident_waves(int qset, int variant, int wpb);  // 0 on error
 int launch_persist(int qset, int variant, int wpb, const PersistArgs &pa, int64_t nwaves,
                    hipStream_t s);
+void launch_zero_u32(unsigned int *p, int n, hipStream_t s);  // p[0..n) = 0, n <= 64
 void launch_comp_mech(int mech, int which, const CompArgs &a, hipStream_t s);
 
 }  // namespace gckpp

@@ -1128,6 +1128,30 @@ __device__ __forceinline__ void lane_finish(const IntegrateArgs &a, const Slot &
   if (a.ierr) a.ierr[cell] = S.ierr;
 }
 
+// Entry idx of the hand-off list: the cell and its state at a step boundary
+// (S.newStep: rejLast = rejMore = false, IERR 1, Hnew recomputed by the next
+// attempt).  The fixed species and the rate constants are not saved: the
+// integration does not change them, and the resuming kernel reads them from
+// conc and rconst as lane_begin did.
+template <class M>
+__device__ __forceinline__ void lane_handoff(const Handoff &ho, const Slot &sl, int64_t cell,
+                                             int64_t idx, const LaneState &S) {
+  using L = SlotLayout<M>;
+  const BCol Y = sl.col(L::Y);
+  double *const r = ho.rec + idx;
+  const int64_t cap = ho.cap;
+  r[0] = S.T;
+  r[cap] = S.H;
+  r[2 * cap] = S.Hexit;
+  r[3 * cap] = S.Texit;
+  r[4 * cap] = S.HnewAcc;
+  for (int k = 0; k < 8; ++k) r[(5 + k) * cap] = (double)S.st[k];
+  r[13 * cap] = (double)S.tries;
+  #pragma unroll 4
+  for (int i = 0; i < M::NVAR; ++i) r[(HO_FIELDS + i) * cap] = Y[i];
+  ho.cell[idx] = (int32_t)cell;
+}
+
 // Calls f(tag) with tag a null pointer of the queue's mechanism type.
 template <class M0, class M1, class M2, class Fn>
 __device__ __forceinline__ void on_queue(int q, Fn &&f) {
@@ -1189,6 +1213,24 @@ __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
         lane_begin<M>(pa.q[q], sl, cell, false, S);
       });
     }
+    if constexpr (NQ == 1 && !IsEuler<M0>::value) {
+      // tail hand-off: at most ho.cap cells of the launch left unfinished
+      // (a cell whose integration has reached Tend finishes here instead)
+      const RosParams &P = pa.q[0].prm;
+      if (pa.ho.cap > 0 && S.newStep &&
+          (P.direction > 0 ? (S.T - P.Tend) + P.roundoff <= 0.0
+                           : (P.Tend - S.T) + P.roundoff <= 0.0) &&
+          total - (int64_t)__hip_atomic_load(pa.ho.done, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) <= pa.ho.cap) {
+        const int64_t idx = (int64_t)atomicAdd(pa.ho.count, 1u);
+        if (idx < pa.ho.cap) {
+          lane_handoff<M0>(pa.ho, sl, cell, idx, S);
+          atomicAdd(pa.ho.done, 1u);
+          cell = -1;
+          continue;
+        }
+      }
+    }
     bool done = false;
     on_queue<M0, M1, M2>(q, [&](auto *tag) {
       using M = std::remove_pointer_t<decltype(tag)>;
@@ -1212,6 +1254,7 @@ __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
           using M = std::remove_pointer_t<decltype(tag)>;
           lane_finish<M>(a, sl, cell, S);
         });
+        if (NQ == 1 && pa.ho.cap > 0) atomicAdd(pa.ho.done, 1u);
         cell = -1;
       }
     }
@@ -1260,6 +1303,9 @@ __global__ __launch_bounds__(64) void k_euler(IntegrateArgs a) {
 
 __global__ void k_zero_u32(unsigned int *p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0u;
+}
+__global__ void k_zero_n_u32(unsigned int *p, int n) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 // Component kernels (parity checks) ------------------------------------------
@@ -1403,6 +1449,9 @@ int launch_persist(int qset, int variant, int wpb, const PersistArgs &pa, int64_
   return hipLaunchKernel(k, dim3((unsigned)(nwaves / wpb)), dim3(64 * wpb), args, lds, s) ==
                  hipSuccess
              ? 0 : -1;
+}
+void launch_zero_u32(unsigned int *p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_n_u32, dim3(1), dim3(64), 0, s, p, n);
 }
 void launch_comp_mech(int mech, int which, const CompArgs &a, hipStream_t s) {
   if (mech == MECH_FULLCHEM) launch_comp<Traits_fullchem>(which, a, s);

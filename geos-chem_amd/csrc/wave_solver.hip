@@ -407,6 +407,39 @@ __device__ void wave_euler(const IntegrateArgs &a, const WaveCell<M> &c, WaveSta
   S.ierr = 1;
 }
 
+// Continues a cell the lane kernel handed off at a step boundary (entry idx
+// of the hand-off list; see Handoff): Y of the variable species and the
+// integrator state from the list, the fixed species and the rate constants
+// from conc and rconst as wave_begin reads them.
+template <class M>
+__device__ void wave_resume(const IntegrateArgs &a, const Handoff &ho, const WaveCell<M> &c,
+                            int64_t cell, int64_t idx, WaveState &S) {
+  double *sh = c.sh;
+  double *rct = c.rct();
+  const double *r = ho.rec + idx;
+  const int64_t cap = ho.cap;
+#pragma unroll 1
+  for (int s = c.lane; s < M::NSPEC; s += 64)
+    sh[M::WV_Y0 + s] = s < M::NVAR ? r[(HO_FIELDS + s) * cap] : a.conc[(int64_t)s * a.cstride + cell];
+#pragma unroll 1
+  for (int k = c.lane; k < M::NREACT; k += 64) rct[k] = a.rconst[(int64_t)k * a.rstride + cell];
+#pragma unroll 1
+  for (int i = c.lane; i < M::WV_NCOEF; i += 64) sh[M::WV_COEF0 + i] = M::wv_coef(i);
+  if (c.lane == 0) sh[M::WV_ZERO] = 0.0;
+  wave_sync_global();
+  S.T = r[0];
+  S.H = r[cap];
+  S.Hnew = S.H;
+  S.Hexit = r[2 * cap];
+  S.Texit = r[3 * cap];
+  S.HnewAcc = r[4 * cap];
+  for (int k = 0; k < 8; ++k) S.st[k] = (int)r[(5 + k) * cap];
+  S.tries = (int)r[13 * cap];
+  S.ierr = 1;
+  S.rejLast = S.rejMore = false;
+  S.newStep = true;
+}
+
 template <class M>
 __device__ void wave_finish(const IntegrateArgs &a, const WaveCell<M> &c, int64_t cell,
                             const WaveState &S) {
@@ -463,7 +496,13 @@ __global__ __launch_bounds__(64) void k_wave(PersistArgs pa) {
   const int64_t wave = blockIdx.x;
   double *g = pa.work + wave * (64 * pa.slot_fields);
   constexpr int NQ = std::is_void_v<M1> ? 1 : (std::is_void_v<M2> ? 2 : 3);
-  const int64_t total = pa.qend[NQ - 1];
+  int64_t total = pa.qend[NQ - 1];
+  if constexpr (NQ == 1 && !IsEulerW<M0>::value) {
+    if (pa.ho.resume) {  // the cells a lane kernel handed off before this launch
+      const int64_t n = (int64_t)pa.ho.count[0];
+      total = n < pa.ho.cap ? n : pa.ho.cap;
+    }
+  }
   int64_t cell = -1;
   int q = 0;
   int64_t next_static = wave;
@@ -480,23 +519,35 @@ __global__ __launch_bounds__(64) void k_wave(PersistArgs pa) {
         next_static += gridDim.x;
       }
       if (cidx >= total) break;
-      // queue of the cell: the number of queue ends at or below cidx (the
-      // ends are nondecreasing).  Written as a sum of compares: the
-      // equivalent early-exit loop was compiled for this wave-uniform cidx
-      // into a select chain that gave the first queue for cells of the
-      // second.
-      q = 0;
+      bool resumed = false;
+      if constexpr (NQ == 1 && !IsEulerW<M0>::value) {
+        if (pa.ho.resume) {
+          q = 0;
+          cell = pa.ho.cell[cidx];
+          wave_resume<M0>(pa.q[0], pa.ho, WaveCell<M0>{sh, g, lane}, cell, cidx, S);
+          resumed = true;
+        }
+      }
+      if (!resumed) {
+        // queue of the cell: the number of queue ends at or below cidx (the
+        // ends are nondecreasing).  Written as a sum of compares: the
+        // equivalent early-exit loop was compiled for this wave-uniform cidx
+        // into a select chain that gave the first queue for cells of the
+        // second.
+        q = 0;
 #pragma unroll
-      for (int k = 0; k < NQ - 1; ++k) q += cidx >= pa.qend[k] ? 1 : 0;
-      cell = cidx - (q ? pa.qend[q - 1] : 0);
-      if (pa.q[q].order) cell = pa.q[q].order[cell];
-      S.tries = 0;
-      on_queue_w<M0, M1, M2>(q, [&](auto *tag) {
-        using M = std::remove_pointer_t<decltype(tag)>;
-        wave_begin<M>(pa.q[q], WaveCell<M>{sh, g, lane}, cell, false, S);
-      });
+        for (int k = 0; k < NQ - 1; ++k) q += cidx >= pa.qend[k] ? 1 : 0;
+        cell = cidx - (q ? pa.qend[q - 1] : 0);
+        if (pa.q[q].order) cell = pa.q[q].order[cell];
+        S.tries = 0;
+        on_queue_w<M0, M1, M2>(q, [&](auto *tag) {
+          using M = std::remove_pointer_t<decltype(tag)>;
+          wave_begin<M>(pa.q[q], WaveCell<M>{sh, g, lane}, cell, false, S);
+        });
+      }
     }
     bool done = false;
+    const int nstp0 = S.st[2];
     on_queue_w<M0, M1, M2>(q, [&](auto *tag) {
       using M = std::remove_pointer_t<decltype(tag)>;
       const WaveCell<M> c{sh, g, lane};
@@ -507,6 +558,8 @@ __global__ __launch_bounds__(64) void k_wave(PersistArgs pa) {
         done = wave_attempt<M>(pa.q[q], c, cell, S);
       }
     });
+    if (pa.ho.resume && lane == 0 && S.st[2] != nstp0)
+      atomicAdd(pa.ho.attempts, (unsigned)(S.st[2] - nstp0));
     if (done) {
       const IntegrateArgs &a = pa.q[q];
       if (S.ierr < 0 && a.retry && S.tries == 0) {

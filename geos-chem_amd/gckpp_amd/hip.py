@@ -31,6 +31,8 @@ SYMBOLS = [
     "gckpp_mlp_grad_dev", "gckpp_mlp_train_step_dev", "gckpp_mlp_finalize",
     "gckpp_order_by_class_dev", "gckpp_order_by_h0_dev", "gckpp_order_by_h0_scratch",
     "gckpp_set_persistent_split",
+    "gckpp_set_persistent_handoff",
+    "gckpp_persistent_handoff_stats",
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
@@ -102,6 +104,9 @@ def load():
     L.gckpp_set_cell_order.argtypes = [vp, vp]
     L.gckpp_set_step_log.argtypes = [vp, vp, ctypes.c_int]
     L.gckpp_set_persistent_split.argtypes = [vp, i64]
+    L.gckpp_set_persistent_handoff.argtypes = [vp, i64]
+    L.gckpp_persistent_handoff_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
+                                                 ctypes.POINTER(ctypes.c_float)]
     L.gckpp_order_by_h0_dev.argtypes = [i64, vp, vp, vp, vp]
     L.gckpp_order_by_h0_scratch.argtypes = [i64]
     fp = ctypes.POINTER(ctypes.c_float)
@@ -216,6 +221,23 @@ class Context:
         kernel on a second stream, concurrently with the lane kernel on the
         rest (0: off); see gckpp_set_persistent_split."""
         _check(self.L.gckpp_set_persistent_split(self.h, int(nfirst)), "gckpp_set_persistent_split")
+
+    def set_persistent_handoff(self, max_cells):
+        """Once at most max_cells cells of a lane-kernel launch are
+        unfinished, they move at a step boundary to a cell-per-wave launch
+        that follows it (0: off); see gckpp_set_persistent_handoff."""
+        _check(self.L.gckpp_set_persistent_handoff(self.h, int(max_cells)),
+               "gckpp_set_persistent_handoff")
+
+    def handoff_stats(self):
+        """(cells handed over, attempts the cell-per-wave launch made on them,
+        lane launch ms, cell-per-wave launch ms) of the last persistent solve
+        with a hand-off."""
+        n = (ctypes.c_int64 * 2)()
+        ms = (ctypes.c_float * 2)()
+        _check(self.L.gckpp_persistent_handoff_stats(self.h, n, ms),
+               "gckpp_persistent_handoff_stats")
+        return int(n[0]), int(n[1]), float(ms[0]), float(ms[1])
 
     def set_step_log(self, hlog, nattempt=0):
         """Device float tensor [2*nattempt][ncell] (or None) receiving (H, err)

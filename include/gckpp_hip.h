@@ -181,6 +181,27 @@ int gckpp_set_step_log(gckpp_ctx *ctx, float *hlog, int nattempt);
  * The first call with nfirst > 0 allocates the cell-per-wave slots. */
 int gckpp_set_persistent_split(gckpp_ctx *ctx, int64_t nfirst);
 
+/* Tail hand-off (no reference counterpart).  With max_cells > 0, a
+ * persistent Integrate or step on a lane variant hands its last cells to the
+ * cell-per-wave kernel: once at most max_cells cells of the lane launch are
+ * unfinished, each of them stops at its next step boundary (before the
+ * first attempt of an integration step) and saves its state -- T, H, the
+ * ISTATUS counters, the last accepted step, Y -- and a cell-per-wave launch
+ * that follows on the same stream continues them.  A lone cell's attempt
+ * takes far less time on the cell-per-wave kernel than on a lane whose wave
+ * is otherwise idle, so the last cells stop setting the step's length.
+ * Results are bitwise those without the hand-off.  0 turns it off; not for
+ * carbon.  A call with a larger max_cells than before allocates the list
+ * ((14 + NVAR) doubles and one int per cell) and, the first time, the
+ * cell-per-wave scratch. */
+int gckpp_set_persistent_handoff(gckpp_ctx *ctx, int64_t max_cells);
+/* Statistics of the last persistent solve with a hand-off (zeros before
+ * the first): n[0] cells handed over, n[1] attempts the cell-per-wave
+ * launch made on them; ms[0] the lane launch's duration, ms[1] the
+ * cell-per-wave launch's (HIP events on the call's stream).  Synchronises
+ * the device. */
+int gckpp_persistent_handoff_stats(gckpp_ctx *ctx, int64_t n[2], float ms[2]);
+
 /* Autotuner: integrates a copy of the given cells with every persistent
  * configuration, keeps the fastest for this context and writes the times
  * [ms] to ms[variant*3 + log2(waves_per_group)] (-1: does not fit).  conc is

@@ -49,6 +49,14 @@ MODULE gckpp_hip_mod
       TYPE(C_PTR),    VALUE :: ctx
       INTEGER(C_INT), VALUE :: solver
     END FUNCTION
+    ! Persistent solver: the last max_cells unfinished cells of a step move
+    ! to the cell-per-wave kernel at a step boundary (0: off)
+    INTEGER(C_INT) FUNCTION gckpp_set_persistent_handoff(ctx, max_cells) &
+         BIND(C, name="gckpp_set_persistent_handoff")
+      IMPORT :: C_INT, C_INT64_T, C_PTR
+      TYPE(C_PTR),        VALUE :: ctx
+      INTEGER(C_INT64_T), VALUE :: max_cells
+    END FUNCTION
     ! One chemistry step of ncell cells (device pointers, HIP stream)
     INTEGER(C_INT) FUNCTION gckpp_fullchem_step_dev(ctx, ncell, dt, icntrl, rcntrl, met, photol, &
          het, het_stride, hstart, conc, hnew, istatus, ierr, stream)                            &

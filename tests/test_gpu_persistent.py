@@ -392,3 +392,59 @@ def test_split_persistent_step_bitwise(name, forced, mech_of, torch_dev):
         assert (out[0][3] < 0).sum() > 20 and (out[0][3] == 1).sum() > 20
     for x, y in zip(out[0], out[150]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("name,forced,variant", [("fullchem", False, 4), ("fullchem", True, 4),
+                                                 ("fullchem", False, 8), ("Hg", False, 1)])
+def test_handoff_persistent_step_bitwise(name, forced, variant, mech_of, torch_dev):
+    """The tail hand-off -- the lane kernel's last unfinished cells saved at
+    a step boundary and continued by the cell-per-wave kernel -- gives
+    bitwise the results without it, for the step (with forced retries, so
+    that cells are handed off in their second integration too) and for
+    Integrate alone, also together with the split solve; the hand-off list
+    is used and never overflows."""
+    print("case: hand-off, %s, forced retries %s, variant %d" % (name, forced, variant), flush=True)
+    from gckpp_amd import hip, synth
+    from oracle.ref import icntrl_for
+    T = torch_dev
+    m = mech_of(name)
+    w = synth.box_cells(m, 1024, 9)
+    ic, rc = icntrl_for(name)
+    hs = w.hstart
+    if forced:
+        ic = ic.copy()
+        ic[3] = 40
+        hs = np.full(w.ncell, 1e-13)
+    order = hip.order_by_h0(_t(T, 10.0 ** np.random.default_rng(2).uniform(-5, 3, w.ncell)))
+    out, counts = {}, {}
+    for split, hand in ((0, 0), (0, 400), (150, 400), (0, 1024)):
+        ctx = hip.Context(m, 2048)
+        ctx.set_solver(hip.SOLVER_PERSISTENT)
+        ctx.set_persistent_config(variant, 1)
+        ctx.set_cell_order(order)
+        ctx.set_persistent_split(split)
+        ctx.set_persistent_handoff(hand)
+        conc = _t(T, w.conc)
+        hnew, ist, ierr, ok = ctx.fullchem_step(_t(T, w.met), _t(T, w.photol), _t(T, w.het), conc,
+                                                _t(T, hs), 1200.0, ic, rc)
+        res = (conc.cpu().numpy(), hnew.cpu().numpy(), ist.cpu().numpy(), ierr.cpu().numpy())
+        stats = ctx.handoff_stats()
+        counts[(split, hand)] = stats[0]
+        if hand:   # the cell-per-wave launch attempted the handed-off cells; both launches timed
+            assert stats[0] > 0 and stats[1] > 0 and stats[2] > 0 and stats[3] > 0, stats
+        c2 = _t(T, w.conc)
+        rct = ctx.rates(_t(T, w.met), _t(T, w.photol), _t(T, w.het), c2)
+        ist2, rst2, ierr2 = ctx.integrate(c2, rct, _t(T, hs), 0.0, 1200.0, ic, rc)
+        out[(split, hand)] = res + (c2.cpu().numpy(), ist2.cpu().numpy(), rst2.cpu().numpy(),
+                                    ierr2.cpu().numpy())
+        ctx.close()
+    print("handed off:", counts, flush=True)
+    if forced:
+        assert (out[(0, 0)][3] < 0).sum() > 20 and (out[(0, 0)][3] == 1).sum() > 20
+    assert counts[(0, 0)] == 0
+    for key in ((0, 400), (150, 400), (0, 1024)):
+        assert 0 < counts[key] <= key[1]
+        for x, y in zip(out[(0, 0)], out[key]):
+            assert np.array_equal(x, y), key
+    # every cell unfinished from the start: all handed off at their first step
+    assert counts[(0, 1024)] == 1024
