@@ -62,6 +62,8 @@ def parse():
     p.add_argument("--handoff", type=int, default=32768,
                    help="persistent solver: once at most this many cells of a lane-kernel launch "
                         "are unfinished, they move to the cell-per-wave kernel (0: off)")
+    p.add_argument("--max-waves", type=int, default=0,
+                   help="cap on the waves of a persistent launch (0: every resident wave)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device copy (reference for HBM byte counters)")
     return p.parse_args()
@@ -192,6 +194,7 @@ def grid_bench(args):
     # than the resident lanes would time a different, latency-only regime),
     # with the tail hand-off it will run with
     ctx.set_persistent_handoff(args.handoff)
+    ctx.set_persistent_waves(args.max_waves)
     tune_ms = None
     if args.variant >= 0:
         ctx.set_persistent_config(args.variant, args.wpb)
@@ -447,6 +450,7 @@ def mixed_bench(args):
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     ctxs = [hip.Context(m, p.ncell) for m, p in zip(mechs, parts)]
     B = hip.Batch(*ctxs)
+    B.set_handoff(args.handoff)
     state = []
     for m, p in zip(mechs, parts):
         ic, rc = icntrl_for(m.name)
@@ -512,6 +516,7 @@ def mixed_bench(args):
                                "HIP graph, warm start from RSTATUS(3); between steps (untimed) a "
                                "synthetic emission/transport perturbation of the state" % [p.ncell for p in parts],
                    "persistent_config": {"variant": best[0], "waves_per_group": best[1]},
+                   "handoff_max_cells": args.handoff,
                    "autotune_first_step_ms": {"v%d_wpb%d" % k: round(v * 1e3, 1) for k, v in tune.items()}},
         "failed_cells_last_step": fails,
         "mean_attempts_per_step": {"first": attempts[0], "last": attempts[-1],
@@ -556,6 +561,7 @@ def c90_bench(args):
     if args.variant >= 0:
         ctx.set_persistent_config(args.variant, args.wpb)
     ctx.set_persistent_handoff(args.handoff)
+    ctx.set_persistent_waves(args.max_waves)
     mpath = learned.default_model_path("fullchem")
     model = learned.StepModel(mech).load(mpath) if mpath.exists() else None
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()

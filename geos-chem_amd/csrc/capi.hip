@@ -64,6 +64,7 @@ struct gckpp_ctx {
   int pvariant;        // persistent kernel variant (loads in flight x register budget)
   int pwpb;            // waves per workgroup of the persistent launch
   int64_t pwaves;      // waves of one persistent launch (multiple of pwpb)
+  int64_t pwaves_max;  // caller's cap on pwaves (0: every resident wave)
   // scratch for the host-pointer and retry paths
   std::vector<double> h_atol, h_rtol;
   double *hs_d;        // host-pointer calls: device staging of one chunk (allocated on use)
@@ -119,6 +120,7 @@ static int set_pconfig(gckpp_ctx *c, int variant, int wpb) {
   c->pvariant = variant;
   c->pwpb = wpb;
   c->pwaves = res < cap ? res / wpb * wpb : cap;
+  if (c->pwaves_max > 0 && c->pwaves > c->pwaves_max) c->pwaves = c->pwaves_max / wpb * wpb;
   if (c->pwaves <= 0) c->pwaves = wpb;
   return 0;
 }
@@ -885,6 +887,15 @@ extern "C" int gckpp_set_persistent_config(gckpp_ctx *c, int variant, int waves_
   return 0;
 }
 
+extern "C" int gckpp_set_persistent_waves(gckpp_ctx *c, int64_t max_waves) {
+  if (!c) return fail("gckpp_set_persistent_waves: null context");
+  if (max_waves < 0) return fail("gckpp_set_persistent_waves: negative wave count");
+  c->pwaves_max = max_waves;
+  if (set_pconfig(c, c->pvariant, c->pwpb) != 0)
+    return fail("gckpp_set_persistent_waves: configuration does not fit");
+  return 0;
+}
+
 extern "C" int gckpp_set_cell_order(gckpp_ctx *c, const int32_t *order) {
   if (!c) return fail("gckpp_set_cell_order: null context");
   c->order = order;
@@ -979,6 +990,13 @@ struct gckpp_batch {
   hipStream_t cap_stream; // stream the graph is captured on
   hipGraph_t graph;
   hipGraphExec_t exec;
+  // tail hand-off (gckpp_batch_set_handoff), as in a context
+  int64_t ho_cap, ho_alloc;
+  unsigned int *ho_ctr;   // [4]
+  int32_t *ho_cell;
+  double *ho_rec;         // [HO_FIELDS + max NVAR][ho_cap]
+  double *hwork;
+  int64_t hwaves, hw_fields;
 };
 
 static int batch_config(gckpp_batch *b, int variant, int wpb, int64_t cap_waves) {
@@ -1069,11 +1087,80 @@ static int batch_enqueue(gckpp_batch *b, const gckpp_part parts[3], double dt, h
   pa.work = b->work;
   pa.slot_fields = b->slot_fields;
   pa.counter = b->counter;
+  const bool hand = b->ho_cap > 0 && b->ho_rec && b->variant != PERSIST_WAVE_VARIANT;
+  if (hand) {
+    launch_zero_u32(b->ho_ctr, 4, s);
+    pa.ho = Handoff{b->ho_ctr, b->ho_ctr + 1, b->ho_ctr + 3, b->ho_cell, b->ho_rec, b->ho_cap, 0};
+  }
   int64_t nw = cells_to_waves(b->variant, end);
   nw = (nw + b->wpb - 1) / b->wpb * b->wpb;
   if (nw > b->nwaves) nw = b->nwaves;
   if (launch_persist(PQ_MIXED, b->variant, b->wpb, pa, nw, s) != 0)
     return fail("gckpp_batch: persistent launch failed: %s", hipGetErrorString(hipGetLastError()));
+  if (hand) {
+    PersistArgs ph = pa;
+    ph.ho.resume = 1;
+    ph.work = b->hwork;
+    ph.slot_fields = b->hw_fields;
+    ph.counter = b->ho_ctr + 2;
+    if (launch_persist(PQ_MIXED, PERSIST_WAVE_VARIANT, 1, ph, b->hwaves, s) != 0)
+      return fail("gckpp_batch: hand-off launch failed: %s", hipGetErrorString(hipGetLastError()));
+  }
+  return 0;
+}
+
+extern "C" int gckpp_batch_set_handoff(gckpp_batch *b, int64_t max_cells) {
+  if (!b) return fail("gckpp_batch_set_handoff: null batch");
+  if (max_cells < 0) return fail("gckpp_batch_set_handoff: negative cell count");
+  if (max_cells > INT32_MAX)
+    return fail("gckpp_batch_set_handoff: %lld cells exceed the list's index range",
+                (long long)max_cells);
+  if (max_cells > b->ho_alloc) {
+    int64_t nvar = 0, hwf = 0;
+    for (int q = 0; q < 3; ++q) {
+      const MechInfo mi = mech_info(q);  // queue q holds mechanism q
+      nvar = mi.nvar > nvar ? mi.nvar : nvar;
+      const int64_t f = (mi.nreact + 5 * (int64_t)mi.nvar + 63) / 64;
+      hwf = f > hwf ? f : hwf;
+    }
+    for (void *p : {(void *)b->ho_cell, (void *)b->ho_rec})
+      if (p) (void)hipFree(p);
+    b->ho_cell = nullptr;
+    b->ho_rec = nullptr;
+    b->ho_cap = b->ho_alloc = 0;
+    hipError_t e = hipMalloc(&b->ho_cell, sizeof(int32_t) * (size_t)max_cells);
+    if (e == hipSuccess)
+      e = hipMalloc(&b->ho_rec, sizeof(double) * (size_t)(HO_FIELDS + nvar) * (size_t)max_cells);
+    if (e == hipSuccess && !b->ho_ctr) e = hipMalloc(&b->ho_ctr, 4 * sizeof(unsigned int));
+    if (e == hipSuccess && !b->hwork) {
+      const int64_t res = persist_resident_waves(PQ_MIXED, PERSIST_WAVE_VARIANT, 1);
+      if (res <= 0) e = hipErrorInvalidConfiguration;
+      b->hw_fields = hwf;
+      b->hwaves = res;
+      if (e == hipSuccess)
+        e = hipMalloc(&b->hwork, sizeof(double) * 64 * (size_t)hwf * (size_t)res);
+    }
+    if (e != hipSuccess) {
+      for (void *p : {(void *)b->ho_cell, (void *)b->ho_rec})
+        if (p) (void)hipFree(p);
+      b->ho_cell = nullptr;
+      b->ho_rec = nullptr;
+      return fail("gckpp_batch_set_handoff: %s", hipGetErrorString(e));
+    }
+    b->ho_alloc = max_cells;
+  }
+  b->ho_cap = max_cells;
+  return 0;
+}
+
+extern "C" int gckpp_batch_handoff_count(gckpp_batch *b, int64_t *count) {
+  if (!b || !count) return fail("gckpp_batch_handoff_count: null argument");
+  *count = 0;
+  if (!b->ho_ctr) return 0;
+  unsigned int n = 0;
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(&n, b->ho_ctr + 1, sizeof n, hipMemcpyDeviceToHost));
+  *count = (int64_t)n;
   return 0;
 }
 
@@ -1131,8 +1218,9 @@ extern "C" int gckpp_batch_finalize(gckpp_batch *b) {
   if (b->exec) ok = hipGraphExecDestroy(b->exec) == hipSuccess && ok;
   if (b->graph) ok = hipGraphDestroy(b->graph) == hipSuccess && ok;
   if (b->cap_stream) ok = hipStreamDestroy(b->cap_stream) == hipSuccess && ok;
-  if (b->work) ok = hipFree(b->work) == hipSuccess && ok;
-  if (b->counter) ok = hipFree(b->counter) == hipSuccess && ok;
+  for (void *p : {(void *)b->work, (void *)b->counter, (void *)b->ho_ctr, (void *)b->ho_cell,
+                  (void *)b->ho_rec, (void *)b->hwork})
+    if (p) ok = hipFree(p) == hipSuccess && ok;
   delete b;
   return ok ? 0 : fail("gckpp_batch_finalize: releasing device resources failed");
 }

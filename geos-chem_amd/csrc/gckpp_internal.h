@@ -53,8 +53,9 @@ struct IntegrateArgs {
   int hlog_n;
 };
 
-// Tail hand-off of a one-queue persistent launch (not the Euler mechanism).
-// The lane kernel, once at most `cap` of its cells are unfinished, stops each
+// Tail hand-off of a persistent launch (cells of the Euler mechanism stay on
+// their lanes).  The lane kernel, once at most `cap` of its cells are
+// unfinished, stops each
 // cell at its next step boundary (before the first attempt of a step, where
 // the integrator state is T, H, the counters and Y) and saves that state here;
 // a cell-per-wave launch after it (resume = 1) continues those cells.  Every
@@ -66,11 +67,11 @@ struct Handoff {
   unsigned int *count;   // cells handed off (entries used: min(count, cap))
   unsigned int *attempts;  // k_wave resume: attempts it made (a statistic)
   int32_t *cell;         // [cap] the handed-off cells
-  double *rec;           // [HO_FIELDS + NVAR][cap] their state, field-major
+  double *rec;           // [HO_FIELDS + max NVAR][cap] their state, field-major
   int64_t cap;           // 0: no hand-off
   int resume;            // k_wave: take the cells of the list and resume them
 };
-constexpr int HO_FIELDS = 14;  // T, H, Hexit, Texit, HnewAcc, ISTATUS(1:8), tries
+constexpr int HO_FIELDS = 15;  // T, H, Hexit, Texit, HnewAcc, ISTATUS(1:8), tries, queue
 
 // One launch of the persistent solver: up to three queues of cells (one per
 // mechanism); global index c < qend[0] is cell c of queue 0, and so on.

@@ -1134,7 +1134,7 @@ __device__ __forceinline__ void lane_finish(const IntegrateArgs &a, const Slot &
 // integration does not change them, and the resuming kernel reads them from
 // conc and rconst as lane_begin did.
 template <class M>
-__device__ __forceinline__ void lane_handoff(const Handoff &ho, const Slot &sl, int64_t cell,
+__device__ __forceinline__ void lane_handoff(const Handoff &ho, const Slot &sl, int64_t cell, int q,
                                              int64_t idx, const LaneState &S) {
   using L = SlotLayout<M>;
   const BCol Y = sl.col(L::Y);
@@ -1147,6 +1147,7 @@ __device__ __forceinline__ void lane_handoff(const Handoff &ho, const Slot &sl, 
   r[4 * cap] = S.HnewAcc;
   for (int k = 0; k < 8; ++k) r[(5 + k) * cap] = (double)S.st[k];
   r[13 * cap] = (double)S.tries;
+  r[14 * cap] = (double)q;
   #pragma unroll 4
   for (int i = 0; i < M::NVAR; ++i) r[(HO_FIELDS + i) * cap] = Y[i];
   ho.cell[idx] = (int32_t)cell;
@@ -1213,22 +1214,30 @@ __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
         lane_begin<M>(pa.q[q], sl, cell, false, S);
       });
     }
-    if constexpr (NQ == 1 && !IsEuler<M0>::value) {
+    if (pa.ho.cap > 0 && S.newStep) {
       // tail hand-off: at most ho.cap cells of the launch left unfinished
       // (a cell whose integration has reached Tend finishes here instead)
-      const RosParams &P = pa.q[0].prm;
-      if (pa.ho.cap > 0 && S.newStep &&
-          (P.direction > 0 ? (S.T - P.Tend) + P.roundoff <= 0.0
-                           : (P.Tend - S.T) + P.roundoff <= 0.0) &&
-          total - (int64_t)__hip_atomic_load(pa.ho.done, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT) <= pa.ho.cap) {
-        const int64_t idx = (int64_t)atomicAdd(pa.ho.count, 1u);
-        if (idx < pa.ho.cap) {
-          lane_handoff<M0>(pa.ho, sl, cell, idx, S);
-          atomicAdd(pa.ho.done, 1u);
-          cell = -1;
-          continue;
+      bool handed = false;
+      on_queue<M0, M1, M2>(q, [&](auto *tag) {
+        using M = std::remove_pointer_t<decltype(tag)>;
+        if constexpr (!IsEuler<M>::value) {
+          const RosParams &P = pa.q[q].prm;
+          if ((P.direction > 0 ? (S.T - P.Tend) + P.roundoff <= 0.0
+                               : (P.Tend - S.T) + P.roundoff <= 0.0) &&
+              total - (int64_t)__hip_atomic_load(pa.ho.done, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) <= pa.ho.cap) {
+            const int64_t idx = (int64_t)atomicAdd(pa.ho.count, 1u);
+            if (idx < pa.ho.cap) {
+              lane_handoff<M>(pa.ho, sl, cell, q, idx, S);
+              atomicAdd(pa.ho.done, 1u);
+              handed = true;
+            }
+          }
         }
+      });
+      if (handed) {
+        cell = -1;
+        continue;
       }
     }
     bool done = false;
@@ -1254,7 +1263,7 @@ __global__ __launch_bounds__(256, W) void k_persist(PersistArgs pa) {
           using M = std::remove_pointer_t<decltype(tag)>;
           lane_finish<M>(a, sl, cell, S);
         });
-        if (NQ == 1 && pa.ho.cap > 0) atomicAdd(pa.ho.done, 1u);
+        if (pa.ho.cap > 0) atomicAdd(pa.ho.done, 1u);
         cell = -1;
       }
     }

@@ -497,11 +497,9 @@ __global__ __launch_bounds__(64) void k_wave(PersistArgs pa) {
   double *g = pa.work + wave * (64 * pa.slot_fields);
   constexpr int NQ = std::is_void_v<M1> ? 1 : (std::is_void_v<M2> ? 2 : 3);
   int64_t total = pa.qend[NQ - 1];
-  if constexpr (NQ == 1 && !IsEulerW<M0>::value) {
-    if (pa.ho.resume) {  // the cells a lane kernel handed off before this launch
-      const int64_t n = (int64_t)pa.ho.count[0];
-      total = n < pa.ho.cap ? n : pa.ho.cap;
-    }
+  if (pa.ho.resume) {  // the cells a lane kernel handed off before this launch
+    const int64_t n = (int64_t)pa.ho.count[0];
+    total = n < pa.ho.cap ? n : pa.ho.cap;
   }
   int64_t cell = -1;
   int q = 0;
@@ -519,16 +517,15 @@ __global__ __launch_bounds__(64) void k_wave(PersistArgs pa) {
         next_static += gridDim.x;
       }
       if (cidx >= total) break;
-      bool resumed = false;
-      if constexpr (NQ == 1 && !IsEulerW<M0>::value) {
-        if (pa.ho.resume) {
-          q = 0;
-          cell = pa.ho.cell[cidx];
-          wave_resume<M0>(pa.q[0], pa.ho, WaveCell<M0>{sh, g, lane}, cell, cidx, S);
-          resumed = true;
-        }
-      }
-      if (!resumed) {
+      if (pa.ho.resume) {
+        q = (int)pa.ho.rec[14 * pa.ho.cap + cidx];
+        cell = pa.ho.cell[cidx];
+        on_queue_w<M0, M1, M2>(q, [&](auto *tag) {
+          using M = std::remove_pointer_t<decltype(tag)>;
+          if constexpr (!IsEulerW<M>::value)
+            wave_resume<M>(pa.q[q], pa.ho, WaveCell<M>{sh, g, lane}, cell, cidx, S);
+        });
+      } else {
         // queue of the cell: the number of queue ends at or below cidx (the
         // ends are nondecreasing).  Written as a sum of compares: the
         // equivalent early-exit loop was compiled for this wave-uniform cidx

@@ -32,6 +32,9 @@ SYMBOLS = [
     "gckpp_order_by_class_dev", "gckpp_order_by_h0_dev", "gckpp_order_by_h0_scratch",
     "gckpp_set_persistent_split",
     "gckpp_set_persistent_handoff",
+    "gckpp_set_persistent_waves",
+    "gckpp_batch_set_handoff",
+    "gckpp_batch_handoff_count",
     "gckpp_persistent_handoff_stats",
     "gckpp_last_error",
 ]
@@ -105,6 +108,9 @@ def load():
     L.gckpp_set_step_log.argtypes = [vp, vp, ctypes.c_int]
     L.gckpp_set_persistent_split.argtypes = [vp, i64]
     L.gckpp_set_persistent_handoff.argtypes = [vp, i64]
+    L.gckpp_set_persistent_waves.argtypes = [vp, i64]
+    L.gckpp_batch_set_handoff.argtypes = [vp, i64]
+    L.gckpp_batch_handoff_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.gckpp_persistent_handoff_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
                                                  ctypes.POINTER(ctypes.c_float)]
     L.gckpp_order_by_h0_dev.argtypes = [i64, vp, vp, vp, vp]
@@ -228,6 +234,11 @@ class Context:
         that follows it (0: off); see gckpp_set_persistent_handoff."""
         _check(self.L.gckpp_set_persistent_handoff(self.h, int(max_cells)),
                "gckpp_set_persistent_handoff")
+
+    def set_persistent_waves(self, max_waves):
+        """At most max_waves resident waves in a persistent launch (0: all)."""
+        _check(self.L.gckpp_set_persistent_waves(self.h, int(max_waves)),
+               "gckpp_set_persistent_waves")
 
     def handoff_stats(self):
         """(cells handed over, attempts the cell-per-wave launch made on them,
@@ -483,6 +494,15 @@ class Batch:
                            for k in range(3)])
         self._parts = arr          # the graph keeps the pointers; keep the tensors alive too
         return arr
+
+    def set_handoff(self, max_cells):
+        """Tail hand-off of the batch launch (0: off)."""
+        _check(self.L.gckpp_batch_set_handoff(self.h, int(max_cells)), "gckpp_batch_set_handoff")
+
+    def handoff_count(self):
+        n = ctypes.c_int64(0)
+        _check(self.L.gckpp_batch_handoff_count(self.h, ctypes.byref(n)), "gckpp_batch_handoff_count")
+        return n.value
 
     def step(self, parts, dt):
         _check(self.L.gckpp_batch_step_dev(self.h, self._arr(parts), float(dt), _stream()),

@@ -181,6 +181,13 @@ int gckpp_set_step_log(gckpp_ctx *ctx, float *hlog, int nattempt);
  * The first call with nfirst > 0 allocates the cell-per-wave slots. */
 int gckpp_set_persistent_split(gckpp_ctx *ctx, int64_t nfirst);
 
+/* Cap on the waves of a persistent launch (no reference counterpart): at
+ * most max_waves resident waves take cells (0: every wave that fits, the
+ * default).  Fewer waves mean fewer concurrent slot blocks in HBM, which
+ * can raise each wave's throughput; kept across gckpp_set_persistent_config
+ * and the autotuner. */
+int gckpp_set_persistent_waves(gckpp_ctx *ctx, int64_t max_waves);
+
 /* Tail hand-off (no reference counterpart).  With max_cells > 0, a
  * persistent Integrate or step on a lane variant hands its last cells to the
  * cell-per-wave kernel: once at most max_cells cells of the lane launch are
@@ -240,6 +247,12 @@ typedef struct {
 int gckpp_batch_init(gckpp_ctx *fullchem, gckpp_ctx *hg, gckpp_ctx *carbon,
                      gckpp_batch **out);
 int gckpp_batch_set_config(gckpp_batch *b, int variant, int waves_per_group);
+/* The tail hand-off of gckpp_set_persistent_handoff for the batch's launch
+ * (fullchem and Hg cells; carbon cells finish on their lanes), captured
+ * with the step; gckpp_batch_handoff_count: cells the last step handed
+ * over (synchronises the device). */
+int gckpp_batch_set_handoff(gckpp_batch *b, int64_t max_cells);
+int gckpp_batch_handoff_count(gckpp_batch *b, int64_t *count);
 int gckpp_batch_step_dev(gckpp_batch *b, const gckpp_part parts[3], double dt,
                          void *stream);
 int gckpp_batch_capture(gckpp_batch *b, const gckpp_part parts[3], double dt);

@@ -448,3 +448,66 @@ def test_handoff_persistent_step_bitwise(name, forced, variant, mech_of, torch_d
             assert np.array_equal(x, y), key
     # every cell unfinished from the start: all handed off at their first step
     assert counts[(0, 1024)] == 1024
+
+
+@pytest.mark.parametrize("hand", [700, 2000])
+def test_mixed_batch_handoff_bitwise_and_graph_replays(hand, fullchem, hg, carbon, torch_dev):
+    """The tail hand-off in the mixed-mechanism batch (fullchem and Hg cells
+    continued by the mixed cell-per-wave kernel, carbon cells finishing on
+    their lanes): one step bitwise equal to the batch step without it; a
+    captured graph with the hand-off replayed three times equals three eager
+    steps without it."""
+    print("case: mixed batch, hand-off %d of 2000 cells, graph replay" % hand, flush=True)
+    from gckpp_amd import hip
+    T = torch_dev
+    inputs = _mixed_inputs(fullchem, hg, carbon)
+    ctxs = [hip.Context(m, 2048) for m, _, _, _ in inputs]
+    B = hip.Batch(*ctxs)
+    B.set_config(4, 1)
+
+    def make_state():
+        st = []
+        for m, w, ic, rc in inputs:
+            st.append(dict(met=_t(T, w.met), photol=_t(T, w.photol), het=_t(T, w.het),
+                           conc=_t(T, w.conc), hstart=_t(T, w.hstart),
+                           istatus=T.zeros((20, w.ncell), dtype=T.int32, device="cuda"),
+                           ierr=T.zeros((w.ncell,), dtype=T.int32, device="cuda")))
+        return st, [B.part(icntrl=inp[2], rcntrl=inp[3], **d) for inp, d in zip(inputs, st)]
+
+    def fetch(st):
+        return [(d["conc"].cpu().numpy(), d["hstart"].cpu().numpy(), d["istatus"].cpu().numpy(),
+                 d["ierr"].cpu().numpy()) for d in st]
+
+    res = {}
+    for h in (0, hand):
+        B.set_handoff(h)
+        st, parts = make_state()
+        B.step(parts, 1200.0)
+        T.cuda.synchronize()
+        res[h] = fetch(st)
+        if h:
+            n = B.handoff_count()
+            print("handed off", n, flush=True)
+            assert 0 < n <= h
+    for ra, rb in zip(res[0], res[hand]):
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x, y)
+    B.set_handoff(0)
+    st_e, parts_e = make_state()
+    for _ in range(3):
+        B.step(parts_e, 1200.0)
+    T.cuda.synchronize()
+    eager = [(d["conc"].cpu().numpy(), d["hstart"].cpu().numpy()) for d in st_e]
+    B.set_handoff(hand)
+    st_g, parts_g = make_state()
+    B.capture(parts_g, 1200.0)
+    for k in range(3):
+        B.replay()
+        T.cuda.synchronize()
+        _ = (st_g[0]["conc"] * 1.0).sum().item()
+    for (ce, he), d in zip(eager, st_g):
+        assert np.array_equal(d["conc"].cpu().numpy(), ce)
+        assert np.array_equal(d["hstart"].cpu().numpy(), he)
+    B.close()
+    for c in ctxs:
+        c.close()
