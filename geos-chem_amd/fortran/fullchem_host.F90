@@ -7,7 +7,8 @@
 !
 ! Usage: fullchem_host <batch file> <result file>
 ! Batch file (stream, native byte order): ncell, max_cells, het columns
-! (int64 each), solver, 0 (int32 each), DT, ICNTRL(20) (int32), RCNTRL(20),
+! (int64 each), solver, tail hand-off cells (0: none; int32 each), DT,
+! ICNTRL(20) (int32), RCNTRL(20),
 ! ATOL(NVAR), RTOL(NVAR), MW(NSPEC), HENRY_K0(NSPEC), HENRY_CR(NSPEC), MET,
 ! PHOTOL, HET, HSTART, C (float64, column-major).
 ! Result file: the step's return code (int32), C, HNEW (float64), ISTATUS,
@@ -16,7 +17,7 @@ PROGRAM fullchem_host
   USE ISO_C_BINDING
   USE gckpp_hip_mod
   IMPLICIT NONE
-  INTEGER(C_INT32_T)              :: sz(8), icntrl(20), solver, pad, rcode
+  INTEGER(C_INT32_T)              :: sz(8), icntrl(20), solver, handoff, rcode
   INTEGER(C_INT64_T)              :: ncell, max_cells, hcols, hstride
   INTEGER                         :: nspec, nvar, nmet, nhet, nphot, u, rc
   REAL(C_DOUBLE)                  :: dt, rcntrl(20)
@@ -34,7 +35,7 @@ PROGRAM fullchem_host
 
   ! gathered batch (phase 1 of Do_FullChem, done by the caller)
   OPEN(NEWUNIT=u, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
-  READ(u) ncell, max_cells, hcols, solver, pad, dt, icntrl, rcntrl
+  READ(u) ncell, max_cells, hcols, solver, handoff, dt, icntrl, rcntrl
   ALLOCATE(atol(nvar), rtol(nvar), mw(nspec), hk0(nspec), hcr(nspec))
   ALLOCATE(met(ncell,nmet), phot(ncell,nphot), het(hcols,nhet), hstart(ncell), c(ncell,nspec))
   ALLOCATE(hnew(ncell), istatus(ncell,20), ierr(ncell))
@@ -50,6 +51,10 @@ PROGRAM fullchem_host
   IF (rc /= 0) CALL die('gckpp_set_species_consts')
   rc = gckpp_set_solver(ctx, solver)
   IF (rc /= 0) CALL die('gckpp_set_solver')
+  IF (handoff > 0) THEN
+    rc = gckpp_set_persistent_handoff(ctx, INT(handoff, C_INT64_T))
+    IF (rc /= 0) CALL die('gckpp_set_persistent_handoff')
+  END IF
 
   ! phase 2: Update_RCONST, Integrate, retry, MAX(C,0) for every cell
   hstride = ncell

@@ -11,7 +11,7 @@ MODULE gckpp_hip_mod
   INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_SOLVER_PLAIN = 0, GCKPP_SOLVER_PERSISTENT = 1
   PUBLIC :: gckpp_init, gckpp_finalize, gckpp_set_tolerances, gckpp_set_species_consts
   PUBLIC :: gckpp_set_solver, gckpp_fullchem_step_dev, gckpp_integrate, gckpp_last_error
-  PUBLIC :: gckpp_mech_sizes, gckpp_rates, gckpp_fullchem_step
+  PUBLIC :: gckpp_mech_sizes, gckpp_rates, gckpp_fullchem_step, gckpp_set_persistent_handoff
 
   INTERFACE
     ! out = NSPEC, NVAR, NFIX, NREACT, LU_NONZERO, NMET, NHET, NPHOTOL (no GPU needed)

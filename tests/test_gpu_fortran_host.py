@@ -27,11 +27,11 @@ def torch_dev():
     return torch
 
 
-def _write_batch(path, m, w, het, max_cells, solver, dt, ic, rc):
+def _write_batch(path, m, w, het, max_cells, solver, dt, ic, rc, handoff=0):
     sc = m.species_consts
     with open(path, "wb") as f:
         f.write(np.array([w.ncell, max_cells, het.shape[1]], dtype=np.int64).tobytes())
-        f.write(np.array([solver, 0], dtype=np.int32).tobytes())
+        f.write(np.array([solver, handoff], dtype=np.int32).tobytes())
         f.write(np.array([dt], dtype=np.float64).tobytes())
         f.write(np.asarray(ic, dtype=np.int32).tobytes())
         f.write(np.asarray(rc, dtype=np.float64).tobytes())
@@ -60,11 +60,15 @@ def _read_result(path, m, n):
     return rcode, conc, hnew, ist, ierr
 
 
-@pytest.mark.parametrize("solver_name,het_per_cell", [("plain", False), ("persistent", True)])
-def test_fortran_host_step_equals_device_step(solver_name, het_per_cell, fullchem, torch_dev,
-                                              tmp_path):
-    print("case: Fortran host step, %s solver, per-cell HetState %s, 600 cells in chunks of 256"
-          % (solver_name, het_per_cell), flush=True)
+@pytest.mark.parametrize("solver_name,het_per_cell,handoff", [("plain", False, 0),
+                                                                ("persistent", True, 0),
+                                                                ("persistent", False, 100)])
+def test_fortran_host_step_equals_device_step(solver_name, het_per_cell, handoff, fullchem,
+                                              torch_dev, tmp_path):
+    """(handoff: the host-pointer calls with the tail hand-off of the last
+    100 cells of each chunk's persistent launch.)"""
+    print("case: Fortran host step, %s solver, per-cell HetState %s, hand-off %d, 600 cells in "
+          "chunks of 256" % (solver_name, het_per_cell, handoff), flush=True)
     from gckpp_amd import hip, synth
     from oracle.ref import fullchem_icntrl
     T = torch_dev
@@ -88,6 +92,7 @@ def test_fortran_host_step_equals_device_step(solver_name, het_per_cell, fullche
     # the host-pointer step through the Python binding, chunked
     ctx2 = hip.Context(m, 256)
     ctx2.set_solver(solver)
+    ctx2.set_persistent_handoff(handoff)
     c2 = np.ascontiguousarray(w.conc.copy())
     h2, i2, e2, ok2 = ctx2.fullchem_step_host(np.ascontiguousarray(w.met),
                                               np.ascontiguousarray(w.photol), het, c2, w.hstart,
@@ -100,7 +105,7 @@ def test_fortran_host_step_equals_device_step(solver_name, het_per_cell, fullche
 
     # the Fortran host, in a child process
     fin, fout = tmp_path / "batch.bin", tmp_path / "result.bin"
-    _write_batch(fin, m, w, het, 256, solver, dt, ic, rc)
+    _write_batch(fin, m, w, het, 256, solver, dt, ic, rc, handoff)
     r = subprocess.run([str(HOST), str(fin), str(fout)], capture_output=True, text=True,
                        timeout=300)
     print(r.stdout, r.stderr)
