@@ -428,6 +428,7 @@ def warm_legs(args, mech, w, ctx, model, P, ic, rc):
     out["learned_split"] = ("cells ordered by learned h0, smallest first; the first %d on the "
                             "cell-per-wave kernel beside the lane kernel" % args.split
                             if args.split > 0 else "one persistent kernel, class order")
+    out["handoff_max_cells"] = args.handoff
     return out
 
 
