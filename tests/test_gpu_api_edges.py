@@ -201,3 +201,29 @@ def test_oversized_context_fails_cleanly(fullchem, torch_dev):
         hip.Context(fullchem, 2_000_000_000)
     ctx = hip.Context(fullchem, 64)
     ctx.close()
+
+
+def test_handoff_and_split_settings_are_checked(fullchem, mech_of, torch_dev):
+    """The tail hand-off and the split take a non-negative cell count, are
+    refused for the carbon mechanism (no Rosenbrock solve), and report zero
+    statistics before any launch used them."""
+    print("case: hand-off and split argument checks", flush=True)
+    from gckpp_amd import hip
+    ctx = hip.Context(fullchem, 64)
+    with pytest.raises(hip.GckppError, match="negative"):
+        ctx.set_persistent_handoff(-1)
+    with pytest.raises(hip.GckppError, match="negative"):
+        ctx.set_persistent_split(-1)
+    with pytest.raises(hip.GckppError, match="negative"):
+        ctx.set_persistent_waves(-1)
+    assert ctx.handoff_stats() == (0, 0, 0.0, 0.0)
+    ctx.set_persistent_handoff(32)
+    assert ctx.handoff_stats() == (0, 0, 0.0, 0.0)
+    ctx.set_persistent_handoff(0)
+    ctx.close()
+    cc = hip.Context(mech_of("carbon"), 64)
+    with pytest.raises(hip.GckppError, match="carbon"):
+        cc.set_persistent_handoff(16)
+    with pytest.raises(hip.GckppError, match="carbon"):
+        cc.set_persistent_split(16)
+    cc.close()
