@@ -469,21 +469,22 @@ def mixed_bench(args):
             s["conc"].copy_(s["conc0"])
             s["hstart"].zero_()
 
-    # autotune: one cold step per configuration
+    # autotune: one cold step per configuration (or the one given)
     tune = {}
-    for v in range(hip.PERSIST_VARIANTS):
-        for g in (1, 2, 4):
-            try:
-                B.set_config(v, g)
-            except hip.GckppError:
-                continue
-            reset()
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            B.step(parts_of(), 1200.0)
-            torch.cuda.synchronize()
-            tune[(v, g)] = time.perf_counter() - t
-            log("mixed config v%d wpb%d: %.1f ms" % (v, g, tune[(v, g)] * 1e3))
+    configs = ([(args.variant, args.wpb)] if args.variant >= 0 else
+               [(v, g) for v in range(hip.PERSIST_VARIANTS) for g in (1, 2, 4)])
+    for v, g in configs:
+        try:
+            B.set_config(v, g)
+        except hip.GckppError:
+            continue
+        reset()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        B.step(parts_of(), 1200.0)
+        torch.cuda.synchronize()
+        tune[(v, g)] = time.perf_counter() - t
+        log("mixed config v%d wpb%d: %.1f ms" % (v, g, tune[(v, g)] * 1e3))
     best = min(tune, key=tune.get)
     B.set_config(*best)
     reset()
