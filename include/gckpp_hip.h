@@ -199,7 +199,7 @@ int gckpp_set_persistent_waves(gckpp_ctx *ctx, int64_t max_waves);
  * is otherwise idle, so the last cells stop setting the step's length.
  * Results are bitwise those without the hand-off.  0 turns it off; not for
  * carbon.  A call with a larger max_cells than before allocates the list
- * ((14 + NVAR) doubles and one int per cell) and, the first time, the
+ * ((15 + NVAR) doubles and one int per cell) and, the first time, the
  * cell-per-wave scratch. */
 int gckpp_set_persistent_handoff(gckpp_ctx *ctx, int64_t max_cells);
 /* Statistics of the last persistent solve with a hand-off (zeros before
