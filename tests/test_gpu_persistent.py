@@ -402,7 +402,8 @@ def test_handoff_persistent_step_bitwise(name, forced, variant, mech_of, torch_d
     bitwise the results without it, for the step (with forced retries, so
     that cells are handed off in their second integration too) and for
     Integrate alone, also together with the split solve; the hand-off list
-    is used and never overflows."""
+    is used and never overflows; the step-size history log is the same too
+    (the cell-per-wave kernel continues a handed-off cell's rows)."""
     print("case: hand-off, %s, forced retries %s, variant %d" % (name, forced, variant), flush=True)
     from gckpp_amd import hip, synth
     from oracle.ref import icntrl_for
@@ -424,10 +425,14 @@ def test_handoff_persistent_step_bitwise(name, forced, variant, mech_of, torch_d
         ctx.set_cell_order(order)
         ctx.set_persistent_split(split)
         ctx.set_persistent_handoff(hand)
+        hlog = T.zeros((48, w.ncell), dtype=T.float32, device="cuda")
+        ctx.set_step_log(hlog, 24)
         conc = _t(T, w.conc)
         hnew, ist, ierr, ok = ctx.fullchem_step(_t(T, w.met), _t(T, w.photol), _t(T, w.het), conc,
                                                 _t(T, hs), 1200.0, ic, rc)
-        res = (conc.cpu().numpy(), hnew.cpu().numpy(), ist.cpu().numpy(), ierr.cpu().numpy())
+        ctx.set_step_log(None)
+        res = (conc.cpu().numpy(), hnew.cpu().numpy(), ist.cpu().numpy(), ierr.cpu().numpy(),
+               hlog.cpu().numpy())
         stats = ctx.handoff_stats()
         counts[(split, hand)] = stats[0]
         if hand:   # the cell-per-wave launch attempted the handed-off cells; both launches timed
