@@ -250,7 +250,9 @@ int gckpp_batch_set_config(gckpp_batch *b, int variant, int waves_per_group);
 /* The tail hand-off of gckpp_set_persistent_handoff for the batch's launch
  * (fullchem and Hg cells; carbon cells finish on their lanes), captured
  * with the step; gckpp_batch_handoff_count: cells the last step handed
- * over (synchronises the device). */
+ * over (synchronises the device).  A captured graph keeps the setting it
+ * was captured with; a max_cells larger than any before reallocates the
+ * list, so capture again after such a call. */
 int gckpp_batch_set_handoff(gckpp_batch *b, int64_t max_cells);
 int gckpp_batch_handoff_count(gckpp_batch *b, int64_t *count);
 int gckpp_batch_step_dev(gckpp_batch *b, const gckpp_part parts[3], double dt,
