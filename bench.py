@@ -62,6 +62,8 @@ def parse():
     p.add_argument("--handoff", type=int, default=32768,
                    help="persistent solver: once at most this many cells of a lane-kernel launch "
                         "are unfinished, they move to the cell-per-wave kernel (0: off)")
+    p.add_argument("--no-e2e", action="store_true",
+                   help="skip the end-to-end leg (host arrays through gckpp_fullchem_step)")
     p.add_argument("--max-waves", type=int, default=0,
                    help="cap on the waves of a persistent launch (0: every resident wave)")
     p.add_argument("--copy-calibration", action="store_true",
@@ -251,6 +253,7 @@ def grid_bench(args):
                           hnew=hnew.clone(), hos=hos)
     ctx.set_cell_order(None)
     L, P = legs["learned"], legs["plain"]
+    e2e = end_to_end(args, w, ctx, ic, rc)
     warm = None if args.no_warm else warm_legs(args, mech, w, ctx, model, P, ic, rc)
 
     # dominant kernel: the persistent integrate (k_persist) of the learned
@@ -337,12 +340,41 @@ def grid_bench(args):
                       "failed_cells": int((P["ierr"] < 0).sum()),
                       "integrate_ms": round(float(np.mean([t[1] for t in P["tim"]])), 3)}},
         "warm_step": warm,
+        "end_to_end": e2e,
         "accuracy": accuracy,
         "autotune_ms": None if tune_ms is None else [[round(float(v), 2) for v in row] for row in tune_ms],
         "cpu_baseline": base,
     }
     ctx.close()
     return out
+
+
+def end_to_end(args, w, ctx, ic, rc):
+    """The same cold step from host arrays through gckpp_fullchem_step (the
+    Fortran host's call): the inputs copied to the device, the step, the
+    results copied back, all inside the timed region.  Persistent solver
+    with the context's hand-off, Hstart = 0 (a host-pointer call takes no
+    cell order or predicted h0), pageable host memory; the first call
+    allocates the staging buffers and is not timed."""
+    import numpy as np
+    from gckpp_amd import hip
+    if args.no_e2e:
+        return None
+    ctx.set_solver(hip.SOLVER_PERSISTENT)
+    ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (w.met, w.photol, w.het)]
+    tim = []
+    for k in range(2):
+        c = np.ascontiguousarray(w.conc, dtype=np.float64).copy()
+        t0 = time.perf_counter()
+        hnew, ist, ierr, ok = ctx.fullchem_step_host(*ins, c, None, w.dt, ic, rc)
+        tim.append(time.perf_counter() - t0)
+    wall = tim[-1]
+    nbytes = sum(a.nbytes for a in ins) + 2 * c.nbytes + hnew.nbytes + ist.nbytes + ierr.nbytes
+    log("end to end (host arrays): %.1f ms" % (wall * 1e3))
+    return {"value": round(w.ncell / wall, 2), "unit": "cells/s", "ms_per_step": round(wall * 1e3, 3),
+            "host_bytes_moved": int(nbytes), "failed_cells": int((ierr < 0).sum()),
+            "path": "host arrays -> gckpp_fullchem_step (staged H2D, persistent solve with the "
+                    "hand-off, D2H), Hstart 0, pageable memory"}
 
 
 def warm_legs(args, mech, w, ctx, model, P, ic, rc):
