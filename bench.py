@@ -167,7 +167,7 @@ def grid_bench(args):
     import numpy as np
     import torch
     from gckpp_amd import hip, learned, mechanism, synth
-    from oracle.ref import fullchem_icntrl
+    from gckpp_amd.options import fullchem_icntrl
 
     mech = mechanism.load("fullchem")
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
@@ -474,7 +474,7 @@ def mixed_bench(args):
     import numpy as np
     import torch
     from gckpp_amd import hip, mechanism, synth
-    from oracle.ref import icntrl_for
+    from gckpp_amd.options import icntrl_for
     mechs = [mechanism.load(k) for k in ("fullchem", "Hg", "carbon")]
     t0 = time.time()
     parts, assign = synth.mixed_grid(mechs, 144, 91, 72, seed=3)
@@ -587,7 +587,7 @@ def c90_bench(args):
     import numpy as np
     import torch
     from gckpp_amd import hip, learned, mechanism, synth
-    from oracle.ref import fullchem_icntrl
+    from gckpp_amd.options import fullchem_icntrl
     mech = mechanism.load("fullchem")
     ic, rc = fullchem_icntrl()
     nface = 90 * 90 * 72

@@ -354,6 +354,7 @@ static int launch_persist_queue(gckpp_ctx *c, PersistArgs &pa, int64_t ncell, hi
   }
   const bool hand = c->ho_cap > 0 && c->ho_rec && c->mech != MECH_CARBON &&
                     c->pvariant != PERSIST_WAVE_VARIANT && ncell - k > 0;
+  c->ho_timed = false;  // the statistics describe this solve from here on
   if (hand) {
     launch_zero_u32(c->ho_ctr, 4, s);
     pa.ho = Handoff{c->ho_ctr, c->ho_ctr + 1, c->ho_ctr + 3, c->ho_cell, c->ho_rec, c->ho_cap, 0};
@@ -398,6 +399,7 @@ static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
   if (code == -100) return fail("gckpp_integrate: %s", why);
   if (code < 0) {
     // every cell fails the same option check, as Rosenbrock() would
+    c->ho_timed = false;
     hipLaunchKernelGGL(k_fill_status, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s,
                        ncell, ierr, code, istatus, rstatus);
     HIP_OK(hipGetLastError());
@@ -430,6 +432,7 @@ static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
     return launch_persist_queue(c, pa, ncell, s, "gckpp_integrate");
   }
   // chunks of at most wstride cells; outputs are indexed with the full ncell
+  if (!only_failed) c->ho_timed = false;  // no hand-off in this solve
   for (int64_t c0 = 0; c0 < ncell; c0 += c->wstride) {
     int64_t n = ncell - c0 < c->wstride ? ncell - c0 : c->wstride;
     IntegrateArgs a{};
@@ -505,44 +508,6 @@ extern "C" int gckpp_solve_dev(gckpp_ctx *c, int64_t ncell, const double *lu, do
   return comp(c, 3, ncell, lu, nullptr, x, nullptr, stream);
 }
 
-// Host-pointer Integrate: copies in, solves, copies out (synchronous).
-extern "C" int gckpp_integrate(gckpp_ctx *c, int64_t ncell, double tin, double tout,
-                               const int32_t icntrl[20], const double rcntrl[20],
-                               const double *hstart, double *conc, const double *rconst,
-                               int32_t *istatus, double *rstatus, int32_t *ierr) {
-  if (!c) return fail("gckpp_integrate: null context");
-  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_integrate: negative ncell");
-  size_t nsp = c->info.nspec, nre = c->info.nreact, n = ncell;
-  double *dc = nullptr, *dr = nullptr, *dh = nullptr, *drs = nullptr;
-  int32_t *dis = nullptr, *die = nullptr;
-  int rc = 0;
-  if (hipMalloc(&dc, 8 * nsp * n) != hipSuccess || hipMalloc(&dr, 8 * nre * n) != hipSuccess ||
-      hipMalloc(&dh, 8 * n) != hipSuccess || hipMalloc(&drs, 8 * 20 * n) != hipSuccess ||
-      hipMalloc(&dis, 4 * 20 * n) != hipSuccess || hipMalloc(&die, 4 * n) != hipSuccess) {
-    rc = fail("gckpp_integrate: device allocation failed");
-  } else {
-    hipError_t e = hipMemcpy(dc, conc, 8 * nsp * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dr, rconst, 8 * nre * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hstart ? hipMemcpy(dh, hstart, 8 * n, hipMemcpyHostToDevice) : hipMemset(dh, 0, 8 * n);
-    if (e != hipSuccess) rc = fail("gckpp_integrate: %s", hipGetErrorString(e));
-    if (rc == 0)
-      rc = gckpp_integrate_dev(c, ncell, tin, tout, icntrl, rcntrl, dh, dc, dr, dis, drs, die,
-                               nullptr);
-    if (rc == 0) {
-      e = hipDeviceSynchronize();
-      if (e == hipSuccess) e = hipMemcpy(conc, dc, 8 * nsp * n, hipMemcpyDeviceToHost);
-      if (e == hipSuccess && istatus) e = hipMemcpy(istatus, dis, 4 * 20 * n, hipMemcpyDeviceToHost);
-      if (e == hipSuccess && rstatus) e = hipMemcpy(rstatus, drs, 8 * 20 * n, hipMemcpyDeviceToHost);
-      if (e == hipSuccess && ierr) e = hipMemcpy(ierr, die, 4 * n, hipMemcpyDeviceToHost);
-      if (e != hipSuccess) rc = fail("gckpp_integrate: %s", hipGetErrorString(e));
-    }
-  }
-  for (void *p : {(void *)dc, (void *)dr, (void *)dh, (void *)drs, (void *)dis, (void *)die})
-    if (p && hipFree(p) != hipSuccess && rc == 0) rc = fail("gckpp_integrate: hipFree failed");
-  return rc;
-}
-
 extern "C" int gckpp_rates_dev(gckpp_ctx *c, int64_t ncell, const double *met,
                                const double *photol, const double *het, int64_t het_stride,
                                const double *conc, double *rconst, void *stream) {
@@ -608,14 +573,13 @@ __global__ __launch_bounds__(256) void k_sum_partials(int64_t nb, const int32_t 
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// The step of gckpp_fullchem_step_dev; *nfail receives the number of cells
-// that failed twice (not an error here).
+// The step of gckpp_fullchem_step_dev, queued on the stream; the number of
+// cells that failed twice goes to the device counter c->nfail (not an error
+// here: gckpp_step_failures reads it).
 static int step_impl(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntrl[20],
                      const double rcntrl[20], const double *met, const double *photol,
                      const double *het, int64_t het_stride, const double *hstart, double *conc,
-                     double *hnew, int32_t *istatus, int32_t *ierr, void *stream,
-                     int32_t *nfail_out) {
-  *nfail_out = 0;
+                     double *hnew, int32_t *istatus, int32_t *ierr, void *stream) {
   if (!c) return fail("gckpp_fullchem_step: null context");
   if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_fullchem_step: negative ncell");
   if (ncell > c->wstride)
@@ -640,6 +604,7 @@ static int step_impl(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntr
     if (code == -100) return fail("gckpp_fullchem_step: %s", why);
     HIP_OK(hipEventRecord(c->ev[2], s));
     if (code < 0) {
+      c->ho_timed = false;
       // every cell fails the same option check (twice, as the retry would):
       // IERR, zero ISTATUS and RSTATUS, then the clip and Hnew = RSTATUS(3)
       // below, as in the plain step
@@ -669,8 +634,6 @@ static int step_impl(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntr
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, partial, c->nfail);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(c->ev[4], s));
-    HIP_OK(hipMemcpyAsync(nfail_out, c->nfail, sizeof *nfail_out, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
     return 0;
   }
   // C_before_integrate (:1026)
@@ -697,8 +660,6 @@ static int step_impl(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntr
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, partial, c->nfail);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(c->ev[4], s));
-  HIP_OK(hipMemcpyAsync(nfail_out, c->nfail, sizeof *nfail_out, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
   return 0;
 }
 
@@ -708,11 +669,17 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
                                        const double *het, int64_t het_stride,
                                        const double *hstart, double *conc, double *hnew,
                                        int32_t *istatus, int32_t *ierr, void *stream) {
-  int32_t nfail = 0;
-  int rc = step_impl(c, ncell, dt, icntrl, rcntrl, met, photol, het, het_stride, hstart, conc,
-                     hnew, istatus, ierr, stream, &nfail);
-  if (rc) return rc;
-  if (nfail > 0) return fail("gckpp_fullchem_step: %d cell(s) failed twice", nfail);
+  return step_impl(c, ncell, dt, icntrl, rcntrl, met, photol, het, het_stride, hstart, conc,
+                   hnew, istatus, ierr, stream);
+}
+
+extern "C" int gckpp_step_failures(gckpp_ctx *c, int64_t *nfail) {
+  if (!c || !nfail) return fail("gckpp_step_failures: null argument");
+  *nfail = 0;
+  int32_t v = 0;
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(&v, c->nfail, sizeof v, hipMemcpyDeviceToHost));
+  *nfail = v;
   return 0;
 }
 
@@ -725,7 +692,7 @@ extern "C" int gckpp_fullchem_step_dev(gckpp_ctx *c, int64_t ncell, double dt,
 static int host_stage(gckpp_ctx *c, const char *who) {
   if (c->hs_d) return 0;
   const size_t n = (size_t)c->wstride;
-  const size_t nd = (size_t)(NMET + NPHOTOL + NHET + c->info.nspec + c->info.nreact + 2) * n;
+  const size_t nd = (size_t)(NMET + NPHOTOL + NHET + c->info.nspec + c->info.nreact + 22) * n;
   const size_t ni = (size_t)21 * n;
   hipError_t e = hipMalloc(&c->hs_d, sizeof(double) * nd);
   if (e == hipSuccess) e = hipMalloc(&c->hs_i, sizeof(int32_t) * ni);
@@ -751,7 +718,7 @@ static hipError_t stage_out(T *h, const T *d, int64_t rows, int64_t n, int64_t n
                      hipMemcpyDeviceToHost);
 }
 struct Stage {
-  double *met, *photol, *het, *conc, *rconst, *hstart, *hnew;
+  double *met, *photol, *het, *conc, *rconst, *hstart, *hnew, *rstatus;
   int32_t *istatus, *ierr;
 };
 static Stage stage_views(const gckpp_ctx *c, int64_t n) {
@@ -763,6 +730,7 @@ static Stage stage_views(const gckpp_ctx *c, int64_t n) {
   v.rconst = v.conc + (size_t)c->info.nspec * n;
   v.hstart = v.rconst + (size_t)c->info.nreact * n;
   v.hnew = v.hstart + n;
+  v.rstatus = v.hnew + n;
   v.istatus = c->hs_i;
   v.ierr = v.istatus + (size_t)20 * n;
   return v;
@@ -802,6 +770,50 @@ extern "C" int gckpp_rates(gckpp_ctx *c, int64_t ncell, const double *met, const
   return 0;
 }
 
+// Host-pointer Integrate: per chunk of at most max_cells cells, copies in,
+// solves, copies out (synchronous), through the context's staging buffers
+// (allocated at its first host-pointer call and reused by later ones).  A
+// cell order or step log set for device calls does not apply here.
+extern "C" int gckpp_integrate(gckpp_ctx *c, int64_t ncell, double tin, double tout,
+                               const int32_t icntrl[20], const double rcntrl[20],
+                               const double *hstart, double *conc, const double *rconst,
+                               int32_t *istatus, double *rstatus, int32_t *ierr) {
+  if (!c) return fail("gckpp_integrate: null context");
+  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_integrate: negative ncell");
+  if (!conc || !rconst) return fail("gckpp_integrate: null conc or rconst");
+  if (host_stage(c, "gckpp_integrate")) return -1;
+  const int32_t *order = c->order;
+  float *hlog = c->hlog;
+  c->order = nullptr;
+  c->hlog = nullptr;
+  int rc = 0;
+  for (int64_t c0 = 0; c0 < ncell && rc == 0; c0 += c->wstride) {
+    const int64_t n = ncell - c0 < c->wstride ? ncell - c0 : c->wstride;
+    const Stage v = stage_views(c, n);
+    hipError_t e = stage_in(v.conc, conc, c->info.nspec, n, ncell, c0);
+    if (e == hipSuccess) e = stage_in(v.rconst, rconst, c->info.nreact, n, ncell, c0);
+    if (e == hipSuccess)
+      e = hstart ? stage_in(v.hstart, hstart, 1, n, ncell, c0)
+                 : hipMemset(v.hstart, 0, sizeof(double) * n);
+    if (e != hipSuccess) {
+      rc = fail("gckpp_integrate: %s", hipGetErrorString(e));
+      break;
+    }
+    rc = gckpp_integrate_dev(c, n, tin, tout, icntrl, rcntrl, v.hstart, v.conc, v.rconst,
+                             v.istatus, v.rstatus, v.ierr, nullptr);
+    if (rc) break;
+    e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = stage_out(conc, v.conc, c->info.nspec, n, ncell, c0);
+    if (e == hipSuccess && istatus) e = stage_out(istatus, v.istatus, 20, n, ncell, c0);
+    if (e == hipSuccess && rstatus) e = stage_out(rstatus, v.rstatus, 20, n, ncell, c0);
+    if (e == hipSuccess && ierr) e = stage_out(ierr, v.ierr, 1, n, ncell, c0);
+    if (e != hipSuccess) rc = fail("gckpp_integrate: %s", hipGetErrorString(e));
+  }
+  c->order = order;
+  c->hlog = hlog;
+  return rc;
+}
+
 static int host_step_chunks(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntrl[20],
                             const double rcntrl[20], const double *met, const double *photol,
                             const double *het, int64_t het_stride, const double *hstart,
@@ -813,10 +825,11 @@ static int host_step_chunks(gckpp_ctx *c, int64_t ncell, double dt, const int32_
     HIP_OK(stage_inputs(v, n, ncell, c0, met, photol, het, het_stride, conc, c->info.nspec));
     if (hstart) HIP_OK(stage_in(v.hstart, hstart, 1, n, ncell, c0));
     else HIP_OK(hipMemset(v.hstart, 0, sizeof(double) * n));
-    int32_t nfail = 0;
     int rc = step_impl(c, n, dt, icntrl, rcntrl, v.met, v.photol, v.het, het_stride ? n : 0,
-                       v.hstart, v.conc, v.hnew, v.istatus, v.ierr, nullptr, &nfail);
+                       v.hstart, v.conc, v.hnew, v.istatus, v.ierr, nullptr);
     if (rc) return rc;
+    int32_t nfail = 0;
+    HIP_OK(hipMemcpy(&nfail, c->nfail, sizeof nfail, hipMemcpyDeviceToHost));
     failed += nfail;
     HIP_OK(stage_out(conc, v.conc, c->info.nspec, n, ncell, c0));
     HIP_OK(stage_out(ierr, v.ierr, 1, n, ncell, c0));
@@ -1109,6 +1122,18 @@ static int batch_enqueue(gckpp_batch *b, const gckpp_part parts[3], double dt, h
   return 0;
 }
 
+// Destroys the batch's captured graph (if any); replay then reports that
+// nothing is captured.  Called before device buffers the graph's kernels
+// point at are released.
+static int batch_drop_graph(gckpp_batch *b) {
+  bool ok = true;
+  if (b->exec) ok = hipGraphExecDestroy(b->exec) == hipSuccess && ok;
+  if (b->graph) ok = hipGraphDestroy(b->graph) == hipSuccess && ok;
+  b->exec = nullptr;
+  b->graph = nullptr;
+  return ok ? 0 : -1;
+}
+
 extern "C" int gckpp_batch_set_handoff(gckpp_batch *b, int64_t max_cells) {
   if (!b) return fail("gckpp_batch_set_handoff: null batch");
   if (max_cells < 0) return fail("gckpp_batch_set_handoff: negative cell count");
@@ -1116,6 +1141,11 @@ extern "C" int gckpp_batch_set_handoff(gckpp_batch *b, int64_t max_cells) {
     return fail("gckpp_batch_set_handoff: %lld cells exceed the list's index range",
                 (long long)max_cells);
   if (max_cells > b->ho_alloc) {
+    // the list is reallocated: a captured graph would still launch kernels
+    // on the old list, so it goes first (and in-flight work is waited for
+    // before the old list is released)
+    if (batch_drop_graph(b) != 0) return fail("gckpp_batch_set_handoff: releasing the graph failed");
+    HIP_OK(hipDeviceSynchronize());
     int64_t nvar = 0, hwf = 0;
     for (int q = 0; q < 3; ++q) {
       const MechInfo mi = mech_info(q);  // queue q holds mechanism q
@@ -1172,14 +1202,7 @@ extern "C" int gckpp_batch_step_dev(gckpp_batch *b, const gckpp_part parts[3], d
 
 extern "C" int gckpp_batch_capture(gckpp_batch *b, const gckpp_part parts[3], double dt) {
   if (!b || !parts) return fail("gckpp_batch_capture: null argument");
-  if (b->exec) {
-    HIP_OK(hipGraphExecDestroy(b->exec));
-    b->exec = nullptr;
-  }
-  if (b->graph) {
-    HIP_OK(hipGraphDestroy(b->graph));
-    b->graph = nullptr;
-  }
+  if (batch_drop_graph(b) != 0) return fail("gckpp_batch_capture: releasing the old graph failed");
   HIP_OK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
   int rc = batch_enqueue(b, parts, dt, b->cap_stream);
   hipGraph_t g = nullptr;
@@ -1214,9 +1237,7 @@ extern "C" int gckpp_batch_set_config(gckpp_batch *b, int variant, int waves_per
 
 extern "C" int gckpp_batch_finalize(gckpp_batch *b) {
   if (!b) return 0;
-  bool ok = true;
-  if (b->exec) ok = hipGraphExecDestroy(b->exec) == hipSuccess && ok;
-  if (b->graph) ok = hipGraphDestroy(b->graph) == hipSuccess && ok;
+  bool ok = batch_drop_graph(b) == 0;
   if (b->cap_stream) ok = hipStreamDestroy(b->cap_stream) == hipSuccess && ok;
   for (void *p : {(void *)b->work, (void *)b->counter, (void *)b->ho_ctr, (void *)b->ho_cell,
                   (void *)b->ho_rec, (void *)b->hwork})

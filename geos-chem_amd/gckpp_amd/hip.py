@@ -20,7 +20,8 @@ SYMBOLS = [
     "gckpp_set_species_consts", "gckpp_rates_dev", "gckpp_rates", "gckpp_integrate_dev",
     "gckpp_integrate", "gckpp_fullchem_step_dev", "gckpp_fullchem_step", "gckpp_fun_dev",
     "gckpp_jac_dev",
-    "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_step_timings", "gckpp_set_solver",
+    "gckpp_decomp_dev", "gckpp_solve_dev", "gckpp_step_timings", "gckpp_step_failures",
+    "gckpp_set_solver",
     "gckpp_set_persistent_config", "gckpp_get_persistent_config", "gckpp_autotune_dev",
     "gckpp_persistent_variants",
     "gckpp_batch_init", "gckpp_batch_set_config", "gckpp_batch_step_dev",
@@ -92,6 +93,7 @@ def load():
     L.gckpp_decomp_dev.argtypes = [vp, i64, dp, i32p, vp]
     L.gckpp_solve_dev.argtypes = [vp, i64, dp, dp, vp]
     L.gckpp_step_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.gckpp_step_failures.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.gckpp_set_solver.argtypes = [vp, ctypes.c_int]
     L.gckpp_persistent_variants.argtypes = []
     L.gckpp_set_persistent_config.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -217,10 +219,15 @@ class Context:
                "gckpp_set_persistent_config")
 
     def set_cell_order(self, order):
-        """Order (device int32 permutation, or None) in which the persistent
-        solver takes cells; the tensor must outlive its use."""
+        """Order (device int32 permutation of 0..n-1, or None) in which the
+        persistent solver takes cells; the tensor must outlive its use.  Later
+        persistent solves refuse a batch whose cell count differs from its
+        length (the kernels would read past its end)."""
+        if order is not None:
+            _want(order, "order", (None,), self.torch.int32)
+        p = _ptr(order)
+        _check(self.L.gckpp_set_cell_order(self.h, p), "gckpp_set_cell_order")
         self._order = order
-        _check(self.L.gckpp_set_cell_order(self.h, _ptr(order)), "gckpp_set_cell_order")
 
     def set_persistent_split(self, nfirst):
         """The first nfirst cells of the cell order run on the cell-per-wave
@@ -251,10 +258,28 @@ class Context:
         return int(n[0]), int(n[1]), float(ms[0]), float(ms[1])
 
     def set_step_log(self, hlog, nattempt=0):
-        """Device float tensor [2*nattempt][ncell] (or None) receiving (H, err)
-        of each cell's first attempts in the next persistent solves."""
+        """Device float32 tensor [2*nattempt][ncell] (or None) receiving (H,
+        err) of each cell's first attempts in the next persistent solves,
+        which refuse a batch of another cell count."""
+        if hlog is not None:
+            if int(nattempt) <= 0:
+                raise GckppError("set_step_log: nattempt must be positive with a log")
+            _want(hlog, "hlog", (2 * int(nattempt), None), self.torch.float32)
+        p = _ptr(hlog)
+        _check(self.L.gckpp_set_step_log(self.h, p, int(nattempt)), "gckpp_set_step_log")
         self._hlog = hlog
-        _check(self.L.gckpp_set_step_log(self.h, _ptr(hlog), int(nattempt)), "gckpp_set_step_log")
+
+    def _check_extras(self, n, who):
+        """The stored cell order and step log index a batch of their own
+        length: refuse a persistent solve of n cells that does not match."""
+        o = getattr(self, "_order", None)
+        if o is not None and o.numel() != n:
+            raise GckppError("%s: the cell order has %d entries, the batch %d cells"
+                             % (who, o.numel(), n))
+        g = getattr(self, "_hlog", None)
+        if g is not None and g.shape[1] != n:
+            raise GckppError("%s: the step log has room for %d cells, the batch has %d"
+                             % (who, g.shape[1], n))
 
     def persistent_config(self):
         """dict(variant, waves_per_group, waves, loads_in_flight,
@@ -277,6 +302,7 @@ class Context:
         _want(conc, "conc", (self.mech.nspec, n), self.torch.float64)
         _want(rconst, "rconst", (self.mech.nreact, n), self.torch.float64)
         _want(hstart, "hstart", (n,), self.torch.float64)
+        self._check_extras(n, "autotune")
         ms = (ctypes.c_float * (3 * nv))()
         ic = np.ascontiguousarray(icntrl, dtype=np.int32)
         rc = np.ascontiguousarray(rcntrl, dtype=np.float64)
@@ -329,6 +355,7 @@ class Context:
         _want(istatus, "istatus", (20, n), T.int32)
         _want(rstatus, "rstatus", (20, n), T.float64)
         _want(ierr, "ierr", (n,), T.int32)
+        self._check_extras(n, "integrate")
         if istatus is None:
             istatus = T.empty((20, n), dtype=T.int32, device=dev)
         if rstatus is None:
@@ -356,6 +383,7 @@ class Context:
         _want(hnew, "hnew", (n,), T.float64)
         _want(istatus, "istatus", (20, n), T.int32)
         _want(ierr, "ierr", (n,), T.int32)
+        self._check_extras(n, "fullchem_step")
         if hnew is None:
             hnew = T.empty((n,), dtype=T.float64, device=dev)
         if istatus is None:
@@ -365,15 +393,18 @@ class Context:
         ic = np.ascontiguousarray(icntrl, dtype=np.int32)
         rc = np.ascontiguousarray(rcntrl, dtype=np.float64)
         hs = 0 if het.shape[1] == 1 else n
-        rcode = self.L.gckpp_fullchem_step_dev(self.h, n, float(dt), ctypes.c_void_p(ic.ctypes.data),
-                                               ctypes.c_void_p(rc.ctypes.data), _ptr(met), _ptr(photol),
-                                               _ptr(het), hs, _ptr(hstart), _ptr(conc), _ptr(hnew),
-                                               _ptr(istatus), _ptr(ierr), _stream())
-        if rcode != 0:
-            msg = self.L.gckpp_last_error().decode()
-            if "failed twice" not in msg:
-                raise GckppError("gckpp_fullchem_step_dev: " + msg)
-        return hnew, istatus, ierr, rcode == 0
+        _check(self.L.gckpp_fullchem_step_dev(self.h, n, float(dt), ctypes.c_void_p(ic.ctypes.data),
+                                              ctypes.c_void_p(rc.ctypes.data), _ptr(met), _ptr(photol),
+                                              _ptr(het), hs, _ptr(hstart), _ptr(conc), _ptr(hnew),
+                                              _ptr(istatus), _ptr(ierr), _stream()),
+               "gckpp_fullchem_step_dev")
+        return hnew, istatus, ierr, self.step_failures() == 0
+
+    def step_failures(self):
+        """Cells of the last fullchem_step that failed twice (synchronises)."""
+        v = ctypes.c_int64(0)
+        _check(self.L.gckpp_step_failures(self.h, ctypes.byref(v)), "gckpp_step_failures")
+        return int(v.value)
 
     def fullchem_step_host(self, met, photol, het, conc, hstart, dt, icntrl, rcntrl):
         """gckpp_fullchem_step on host (numpy) arrays, the Fortran host's call:

@@ -36,7 +36,7 @@ def main():
     a = ap.parse_args()
     import torch
     from gckpp_amd import hip, learned, mechanism, synth
-    from oracle.ref import icntrl_for
+    from gckpp_amd.options import icntrl_for
 
     mech = mechanism.load(a.mech)
     nx, ny, nz = (int(v) for v in a.grid.split("x"))

@@ -90,6 +90,10 @@ int gckpp_integrate_dev(gckpp_ctx *ctx, int64_t ncell, double tin, double tout,
                         const double *hstart, double *conc, const double *rconst,
                         int32_t *istatus, double *rstatus, int32_t *ierr,
                         void *stream);
+/* The same with host arrays, synchronous; any ncell, in chunks of at most
+ * max_cells through the staging buffers of gckpp_rates (allocated once);
+ * istatus, rstatus and ierr may be null (not returned).  The cell order and
+ * step log of gckpp_set_cell_order / gckpp_set_step_log do not apply. */
 int gckpp_integrate(gckpp_ctx *ctx, int64_t ncell, double tin, double tout,
                     const int32_t icntrl[20], const double rcntrl[20],
                     const double *hstart, double *conc, const double *rconst,
@@ -99,17 +103,26 @@ int gckpp_integrate(gckpp_ctx *ctx, int64_t ncell, double tin, double tout,
  * 953-1350): Update_RCONST, Integrate(0,dt), one retry of failed cells with
  * Hstart=0 and the concentrations restored (:1138-1277), then C=max(C,0).
  * hnew[ncell] = RSTATUS(Nhnew), the next step's Hstart (KPPHvalue, :1294).
- * istatus[20][ncell], ierr[ncell]; returns -1 if any cell failed twice. */
+ * istatus[20][ncell], ierr[ncell] (IERR < 0: the cell failed twice).
+ * Asynchronous like every _dev call: returns -1 only for a bad argument or
+ * a launch failure; gckpp_step_failures gives the number of cells of the
+ * last step that failed twice (the condition for which Do_FullChem returns
+ * GC_FAILURE, :1255-1277). */
 int gckpp_fullchem_step_dev(gckpp_ctx *ctx, int64_t ncell, double dt,
                             const int32_t icntrl[20], const double rcntrl[20],
                             const double *met, const double *photol,
                             const double *het, int64_t het_stride,
                             const double *hstart, double *conc, double *hnew,
                             int32_t *istatus, int32_t *ierr, void *stream);
+/* Cells of the context's last gckpp_fullchem_step_dev that failed twice
+ * (no reference counterpart: Do_FullChem stops at the first, :1255-1277).
+ * Synchronises the device. */
+int gckpp_step_failures(gckpp_ctx *ctx, int64_t *nfail);
 /* The same with host arrays, synchronous -- the one call of the host's
  * Do_FullChem (the Fortran arrays C(ncell,NSPEC), MET(ncell,NMET), ...
  * passed as they are); any ncell, in chunks of at most max_cells.  hstart
- * may be null (cold start), hnew and istatus too (not returned).  The cell
+ * may be null (cold start), hnew and istatus too (not returned).  Returns
+ * -1 if any cell failed twice (after writing every output).  The cell
  * order and step log of gckpp_set_cell_order / gckpp_set_step_log do not
  * apply to this call. */
 int gckpp_fullchem_step(gckpp_ctx *ctx, int64_t ncell, double dt,
@@ -202,11 +215,12 @@ int gckpp_set_persistent_waves(gckpp_ctx *ctx, int64_t max_waves);
  * ((15 + NVAR) doubles and one int per cell) and, the first time, the
  * cell-per-wave scratch. */
 int gckpp_set_persistent_handoff(gckpp_ctx *ctx, int64_t max_cells);
-/* Statistics of the last persistent solve with a hand-off (zeros before
- * the first): n[0] cells handed over, n[1] attempts the cell-per-wave
- * launch made on them; ms[0] the lane launch's duration, ms[1] the
- * cell-per-wave launch's (HIP events on the call's stream).  Synchronises
- * the device. */
+/* Statistics of the context's last solve: n[0] cells handed over, n[1]
+ * attempts the cell-per-wave launch made on them; ms[0] the lane launch's
+ * duration, ms[1] the cell-per-wave launch's (HIP events on the call's
+ * stream).  All zeros when the last solve ran no hand-off (plain solver,
+ * hand-off off, carbon, the cell-per-wave variant) or before the first
+ * solve.  Synchronises the device. */
 int gckpp_persistent_handoff_stats(gckpp_ctx *ctx, int64_t n[2], float ms[2]);
 
 /* Autotuner: integrates a copy of the given cells with every persistent
@@ -251,8 +265,10 @@ int gckpp_batch_set_config(gckpp_batch *b, int variant, int waves_per_group);
  * (fullchem and Hg cells; carbon cells finish on their lanes), captured
  * with the step; gckpp_batch_handoff_count: cells the last step handed
  * over (synchronises the device).  A captured graph keeps the setting it
- * was captured with; a max_cells larger than any before reallocates the
- * list, so capture again after such a call. */
+ * was captured with.  A max_cells larger than any before reallocates the
+ * list: the call then waits for the device and destroys the captured
+ * graph, and gckpp_batch_replay fails (-1, nothing captured) until the
+ * next gckpp_batch_capture. */
 int gckpp_batch_set_handoff(gckpp_batch *b, int64_t max_cells);
 int gckpp_batch_handoff_count(gckpp_batch *b, int64_t *count);
 int gckpp_batch_step_dev(gckpp_batch *b, const gckpp_part parts[3], double dt,

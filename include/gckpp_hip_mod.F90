@@ -12,6 +12,7 @@ MODULE gckpp_hip_mod
   PUBLIC :: gckpp_init, gckpp_finalize, gckpp_set_tolerances, gckpp_set_species_consts
   PUBLIC :: gckpp_set_solver, gckpp_fullchem_step_dev, gckpp_integrate, gckpp_last_error
   PUBLIC :: gckpp_mech_sizes, gckpp_rates, gckpp_fullchem_step, gckpp_set_persistent_handoff
+  PUBLIC :: gckpp_step_failures
 
   INTERFACE
     ! out = NSPEC, NVAR, NFIX, NREACT, LU_NONZERO, NMET, NHET, NPHOTOL (no GPU needed)
@@ -57,7 +58,8 @@ MODULE gckpp_hip_mod
       TYPE(C_PTR),        VALUE :: ctx
       INTEGER(C_INT64_T), VALUE :: max_cells
     END FUNCTION
-    ! One chemistry step of ncell cells (device pointers, HIP stream)
+    ! One chemistry step of ncell cells (device pointers, HIP stream);
+    ! returns once queued (gckpp_step_failures: cells that failed twice)
     INTEGER(C_INT) FUNCTION gckpp_fullchem_step_dev(ctx, ncell, dt, icntrl, rcntrl, met, photol, &
          het, het_stride, hstart, conc, hnew, istatus, ierr, stream)                            &
          BIND(C, name="gckpp_fullchem_step_dev")
@@ -68,6 +70,12 @@ MODULE gckpp_hip_mod
       INTEGER(C_INT32_T), INTENT(IN) :: icntrl(20)
       REAL(C_DOUBLE),     INTENT(IN) :: rcntrl(20)
       TYPE(C_PTR),        VALUE      :: met, photol, het, hstart, conc, hnew, istatus, ierr, stream
+    END FUNCTION
+    ! Cells of the last gckpp_fullchem_step_dev that failed twice (synchronises)
+    INTEGER(C_INT) FUNCTION gckpp_step_failures(ctx, nfail) BIND(C, name="gckpp_step_failures")
+      IMPORT :: C_INT, C_INT64_T, C_PTR
+      TYPE(C_PTR),        VALUE       :: ctx
+      INTEGER(C_INT64_T), INTENT(OUT) :: nfail
     END FUNCTION
     ! Update_RCONST with host arrays (synchronous): MET(ncell,NMET),
     ! PHOTOL(ncell,NPHOTOL), HET(ncell or 1,NHET) (het_stride ncell or 0),

@@ -127,31 +127,3 @@ class RefKPP:
                                    int(nthreads))
         return c, ist, hnew, ierr
 
-
-def fullchem_icntrl():
-    """ICNTRL/RCNTRL of fullchem_AR_SetIntegratorOptions
-    (KPP/fullchem/fullchem_AutoReduceFuncs.F90:204-345), autoreduce off."""
-    ic = np.zeros(20, dtype=np.int32)
-    ic[0], ic[2], ic[6], ic[14] = 1, 4, 1, -1
-    rc = np.zeros(20)
-    return ic, rc
-
-
-def hg_icntrl():
-    """ICNTRL/RCNTRL of the Hg chemistry call (GeosCore/mercury_mod.F90:951-969):
-    autonomous, vector tolerances, Rodas3, no adjoint, rates frozen."""
-    ic = np.zeros(20, dtype=np.int32)
-    ic[0], ic[2], ic[6], ic[14] = 1, 4, 1, -1
-    return ic, np.zeros(20)
-
-
-def carbon_icntrl():
-    """ICNTRL of the carbon chemistry call (GeosCore/carbon_gases_mod.F90:454-457):
-    verbose errors, keep negatives, rates frozen."""
-    ic = np.zeros(20, dtype=np.int32)
-    ic[0], ic[14] = 1, -1
-    return ic, np.zeros(20)
-
-
-def icntrl_for(mech_name):
-    return {"fullchem": fullchem_icntrl, "Hg": hg_icntrl, "carbon": carbon_icntrl}[mech_name]()

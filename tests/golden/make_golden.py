@@ -29,7 +29,7 @@ ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "geos-chem_amd"))
 
-from gckpp_amd import mechanism, synth  # noqa: E402
+from gckpp_amd import mechanism, options, synth  # noqa: E402
 from oracle import ref  # noqa: E402
 
 
@@ -39,7 +39,7 @@ def make(mech_name, cells, het_seed=None):
     w = synth.box_cells(m, 4096, 0).subset(cells)
     if het_seed is not None:
         w.het = synth.rich_het(np.random.default_rng(het_seed), len(cells))
-    ic, rc = ref.icntrl_for(mech_name)
+    ic, rc = options.icntrl_for(mech_name)
     rct = R.rates(w)
     vdot = R.fun(w.conc, rct)
     jvs = R.jac(w.conc, rct)
@@ -76,7 +76,7 @@ def make_carbon(cells):
     m = mechanism.load("carbon")
     R = ref.RefKPP(m)
     w = synth.box_cells(m, 4096, 0).subset(cells)
-    ic, rc = ref.carbon_icntrl()
+    ic, rc = options.carbon_icntrl()
     rct = R.rates(w)
     vdot = R.fun(w.conc, rct)
     c1, _, _, e1 = R.integrate(w.conc, rct, w.hstart, 0.0, 1200.0, ic, rc)

@@ -255,3 +255,33 @@ def test_binding_refuses_host_strided_and_misshaped_tensors():
         hip._want(t, "ierr", (3, 8), torch.int32)
     with pytest.raises(hip.GckppError, match="expected shape"):
         hip._want(t[:, ::2], "x", (3, 8), torch.float64)
+
+
+def test_binding_refuses_mismatched_cell_order_and_step_log():
+    """A stored cell order or step log indexes the batch it was made for: the
+    binding refuses a wrong dtype or rank when it is set, and a persistent
+    solve of another cell count afterwards, before any HIP call (checked on a
+    context shell that holds no device state)."""
+    import torch
+    from gckpp_amd import hip
+    ctx = object.__new__(hip.Context)
+    ctx.torch = torch
+    with pytest.raises(hip.GckppError, match="order: expected torch.int32"):
+        ctx.set_cell_order(torch.zeros(8, dtype=torch.int64))
+    with pytest.raises(hip.GckppError, match="order: expected shape"):
+        ctx.set_cell_order(torch.zeros((2, 4), dtype=torch.int32))
+    with pytest.raises(hip.GckppError, match="hlog: expected shape"):
+        ctx.set_step_log(torch.zeros((6, 8), dtype=torch.float32), 4)
+    with pytest.raises(hip.GckppError, match="hlog: expected torch.float32"):
+        ctx.set_step_log(torch.zeros((8, 8), dtype=torch.float64), 4)
+    with pytest.raises(hip.GckppError, match="nattempt"):
+        ctx.set_step_log(torch.zeros((8, 8), dtype=torch.float32), 0)
+    ctx._order = torch.zeros(8, dtype=torch.int32)
+    ctx._hlog = None
+    ctx._check_extras(8, "integrate")
+    with pytest.raises(hip.GckppError, match="cell order has 8 entries, the batch 9"):
+        ctx._check_extras(9, "integrate")
+    ctx._order = None
+    ctx._hlog = torch.zeros((8, 8), dtype=torch.float32)
+    with pytest.raises(hip.GckppError, match="step log has room for 8 cells"):
+        ctx._check_extras(7, "fullchem_step")

@@ -29,7 +29,7 @@ def _t(torch, a, dtype=None):
 @pytest.fixture(scope="module")
 def box(fullchem, torch_dev):
     from gckpp_amd import hip, synth
-    from oracle.ref import fullchem_icntrl
+    from gckpp_amd.options import fullchem_icntrl
     T = torch_dev
     w = synth.box_cells(fullchem, 600, 11)
     ctx = hip.Context(fullchem, 1024)

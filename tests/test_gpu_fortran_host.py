@@ -70,7 +70,7 @@ def test_fortran_host_step_equals_device_step(solver_name, het_per_cell, handoff
     print("case: Fortran host step, %s solver, per-cell HetState %s, hand-off %d, 600 cells in "
           "chunks of 256" % (solver_name, het_per_cell, handoff), flush=True)
     from gckpp_amd import hip, synth
-    from oracle.ref import fullchem_icntrl
+    from gckpp_amd.options import fullchem_icntrl
     T = torch_dev
     assert HOST.exists(), "the Fortran host is built with the library (make -C geos-chem_amd/csrc)"
     m = fullchem

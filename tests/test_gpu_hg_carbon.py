@@ -104,7 +104,7 @@ def _scaled(m, c, ref):
 def test_hg_integrate_matches_reference(hg, hg_gold, torch_dev, solver):
     print("case: Hg Integrate(0,1200), %s solver" % solver, flush=True)
     from gckpp_amd import hip
-    from oracle.ref import hg_icntrl
+    from gckpp_amd.options import hg_icntrl
     g = hg_gold
     ctx = hip.Context(hg, 1024)
     if solver == "persistent":
@@ -126,7 +126,7 @@ def test_hg_step_matches_reference(hg_ctx, hg_gold, hg, torch_dev):
     """The chemistry step of the Hg module (rates, Integrate, retry, clip),
     mercury_mod.F90:1033-1260, against the reference's."""
     print("case: Hg chemistry step on golden cells", flush=True)
-    from oracle.ref import hg_icntrl
+    from gckpp_amd.options import hg_icntrl
     g = hg_gold
     ic, rc = hg_icntrl()
     conc = _t(torch_dev, g["conc"])
@@ -177,7 +177,7 @@ def test_carbon_rates_and_fun_match_reference(carbon_ctx, carbon_gold, carbon, t
 
 def test_carbon_euler_matches_reference(carbon_ctx, carbon_gold, carbon, torch_dev):
     print("case: carbon forward Euler, ICNTRL(16) = 0, 1, 2", flush=True)
-    from oracle.ref import carbon_icntrl
+    from gckpp_amd.options import carbon_icntrl
     g = carbon_gold
     ic, rc = carbon_icntrl()
     n = g["conc"].shape[1]
@@ -197,7 +197,7 @@ def test_carbon_step_matches_reference(carbon_ctx, carbon_gold, carbon, torch_de
     """The carbon chemistry step (carbon_gases_mod.F90:450-604): rates, one
     Euler step, no clipping of negatives."""
     print("case: carbon chemistry step", flush=True)
-    from oracle.ref import carbon_icntrl
+    from gckpp_amd.options import carbon_icntrl
     g = carbon_gold
     ic, rc = carbon_icntrl()
     conc = _t(torch_dev, g["conc"])

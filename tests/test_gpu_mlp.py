@@ -172,7 +172,7 @@ def test_learned_h0_saves_attempts_within_tolerance(fullchem, torch_dev):
     start is used, and the two starts differ there by as much.)"""
     print("case: learned h0 gain, 36x23x24 grids, cold and warm steps", flush=True)
     from gckpp_amd import hip, learned, synth
-    from oracle.ref import fullchem_icntrl
+    from gckpp_amd.options import fullchem_icntrl
     T = torch_dev
     ic, rc = fullchem_icntrl()
     nx, ny, nz = 36, 23, 24

@@ -13,6 +13,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from gckpp_amd import options
+
 pytestmark = pytest.mark.gpu
 
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -93,7 +95,7 @@ def _scaled_err(c, ref, m, atol=1e-2, rtol=5e-3):
 
 def test_integrate_matches_reference(ctx, gold, fullchem, torch_dev):
     print("case: Integrate(0,1200) on golden box cells", flush=True)
-    from oracle.ref import fullchem_icntrl
+    from gckpp_amd.options import fullchem_icntrl
     ic, rc = fullchem_icntrl()
     conc = _t(torch_dev, gold["conc"])
     ist, rst, ierr = ctx.integrate(conc, _t(torch_dev, gold["rconst"]),
@@ -119,7 +121,7 @@ def test_integrate_matches_oracle_on_box_workload(ctx, fullchem, torch_dev):
     w = synth.box_cells(fullchem, 4096, 0).subset(idx)
     R = ref.RefKPP(fullchem)
     rct = R.rates(w)
-    ic, rc = ref.fullchem_icntrl()
+    ic, rc = options.fullchem_icntrl()
     O = orc.Oracle(fullchem)
     c_o, i_o, r_o, e_o = O.integrate(w.conc, rct, w.hstart, 0.0, 1200.0, ic, rc, nthreads=8)
     conc = _t(torch_dev, w.conc)
@@ -144,7 +146,7 @@ def test_persistent_solver_matches_oracle(fullchem, torch_dev, nblocks_cells):
     w = synth.box_cells(fullchem, 4096, 0).subset(np.r_[0:nblocks_cells // 2, 2048:2048 + nblocks_cells // 2])
     R = ref.RefKPP(fullchem)
     rct = R.rates(w)
-    ic, rc = ref.fullchem_icntrl()
+    ic, rc = options.fullchem_icntrl()
     O = orc.Oracle(fullchem)
     c_o, i_o, r_o, e_o = O.integrate(w.conc, rct, w.hstart, 0.0, 1200.0, ic, rc, nthreads=8)
     ctx = hip.Context(fullchem, 4096)

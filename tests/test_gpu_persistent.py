@@ -13,6 +13,8 @@ other; against the oracle the bar is the repo-tolerance scaled error <= 1
 import numpy as np
 import pytest
 
+from gckpp_amd import options
+
 pytestmark = pytest.mark.gpu
 
 
@@ -49,7 +51,7 @@ def box2k(fullchem):
         pytest.skip("oracle/_ref not built")
     w = synth.box_cells(fullchem, 4096, 0).subset(np.r_[0:1000, 2048:3048])
     rct = ref.RefKPP(fullchem).rates(w)
-    ic, rc = ref.fullchem_icntrl()
+    ic, rc = options.fullchem_icntrl()
     c_o, i_o, r_o, e_o = orc.Oracle(fullchem).integrate(w.conc, rct, w.hstart, 0.0, 1200.0, ic, rc,
                                                          nthreads=16)
     return dict(w=w, rct=rct, ic=ic, rc=rc, c=c_o, ist=i_o, ierr=e_o)
@@ -149,7 +151,7 @@ def test_autotuner_picks_a_configuration(fullchem, torch_dev, box2k):
 
 def _mixed_inputs(fullchem, hg, carbon, n=(600, 250, 150)):
     from gckpp_amd import synth
-    from oracle.ref import icntrl_for
+    from gckpp_amd.options import icntrl_for
     parts = []
     for m, k, seed in ((fullchem, n[0], 0), (hg, n[1], 1), (carbon, n[2], 2)):
         w = synth.box_cells(m, 2 * k, seed)
@@ -236,7 +238,7 @@ def test_step_bitwise_equals_lane_per_cell(name, forced, variant, wpb, mech_of, 
     print("case: variant %d (%d waves per group) vs lane-per-cell step, %s, forced retries %s"
           % (variant, wpb, name, forced), flush=True)
     from gckpp_amd import hip, synth
-    from oracle.ref import icntrl_for
+    from gckpp_amd.options import icntrl_for
     T = torch_dev
     m = mech_of(name)
     w = synth.box_cells(m, 1024, 5)
@@ -359,7 +361,7 @@ def test_split_persistent_step_bitwise(name, forced, mech_of, torch_dev):
     when cells fail and take the retry."""
     print("case: split persistent step, %s, forced retries %s" % (name, forced), flush=True)
     from gckpp_amd import hip, synth
-    from oracle.ref import icntrl_for
+    from gckpp_amd.options import icntrl_for
     T = torch_dev
     m = mech_of(name)
     w = synth.box_cells(m, 1024, 9)
@@ -406,7 +408,7 @@ def test_handoff_persistent_step_bitwise(name, forced, variant, mech_of, torch_d
     (the cell-per-wave kernel continues a handed-off cell's rows)."""
     print("case: hand-off, %s, forced retries %s, variant %d" % (name, forced, variant), flush=True)
     from gckpp_amd import hip, synth
-    from oracle.ref import icntrl_for
+    from gckpp_amd.options import icntrl_for
     T = torch_dev
     m = mech_of(name)
     w = synth.box_cells(m, 1024, 9)
@@ -513,6 +515,16 @@ def test_mixed_batch_handoff_bitwise_and_graph_replays(hand, fullchem, hg, carbo
     for (ce, he), d in zip(eager, st_g):
         assert np.array_equal(d["conc"].cpu().numpy(), ce)
         assert np.array_equal(d["hstart"].cpu().numpy(), he)
+    # a larger hand-off reallocates the list the captured graph points at:
+    # the graph is dropped and replay refuses (before any launch) until the
+    # next capture
+    B.set_handoff(4 * hand)
+    with pytest.raises(hip.GckppError, match="nothing captured"):
+        B.replay()
+    B.capture(parts_g, 1200.0)
+    B.replay()
+    T.cuda.synchronize()
+    assert np.isfinite(st_g[0]["conc"].cpu().numpy()).all()
     B.close()
     for c in ctxs:
         c.close()

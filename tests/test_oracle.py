@@ -99,8 +99,14 @@ def test_decomp_matches_reference_and_dense_lu(orc, gold, fullchem):
     Up = np.triu(F)
     assert np.allclose(Lp, L, rtol=1e-9, atol=1e-12 * np.abs(L).max())
     assert np.allclose(Up, U, rtol=1e-9, atol=1e-12 * np.abs(U).max())
-    # the sparse pattern holds all fill-in: no dense entry outside it
-    assert np.abs(L[F == 0.0] - 0.0).max() < 1e-12 * np.abs(L).max() or True
+    # the sparse pattern holds all fill-in: every dense factor entry outside
+    # it (L below, U on and above the diagonal) is an exact zero, since each
+    # of its updates multiplies by an exact zero
+    inside = np.zeros((n, n), dtype=bool)
+    inside[fullchem.lu_irow, fullchem.lu_icol] = True
+    assert not inside.all()
+    assert np.count_nonzero(L[~inside]) == 0
+    assert np.count_nonzero(np.triu(U)[~inside]) == 0
 
 
 def test_solve_matches_reference_and_numpy(orc, gold, fullchem):
@@ -112,7 +118,7 @@ def test_solve_matches_reference_and_numpy(orc, gold, fullchem):
 
 
 def test_integrate_matches_reference(orc, gold, fullchem):
-    from oracle.ref import fullchem_icntrl
+    from gckpp_amd.options import fullchem_icntrl
     ic, rc = fullchem_icntrl()
     c, ist, rst, ierr = orc.integrate(gold["conc"], gold["rconst"], np.zeros(8), 0.0, 1200.0, ic, rc)
     assert np.array_equal(ierr, gold["int_ierr"])

@@ -83,7 +83,7 @@ def test_hg_decomp_solve_match_reference_and_numpy(hg, hg_gold):
 
 
 def test_hg_integrate_matches_reference(hg, hg_gold):
-    from oracle.ref import hg_icntrl
+    from gckpp_amd.options import hg_icntrl
     O = _oracle(hg)
     g = hg_gold
     ic, rc = hg_icntrl()
@@ -131,7 +131,7 @@ def test_carbon_fun_matches_reference(carbon, carbon_gold):
 
 
 def test_carbon_euler_matches_reference_and_closed_form(carbon, carbon_gold):
-    from oracle.ref import carbon_icntrl
+    from gckpp_amd.options import carbon_icntrl
     O = _oracle(carbon)
     g = carbon_gold
     ic, _ = carbon_icntrl()
@@ -147,7 +147,7 @@ def test_carbon_euler_matches_reference_and_closed_form(carbon, carbon_gold):
 
 
 def test_carbon_negative_modes_match_reference(carbon, carbon_gold):
-    from oracle.ref import carbon_icntrl
+    from gckpp_amd.options import carbon_icntrl
     O = _oracle(carbon)
     g = carbon_gold
     ic, _ = carbon_icntrl()
