@@ -62,6 +62,8 @@ def parse():
     p.add_argument("--handoff", type=int, default=32768,
                    help="persistent solver: once at most this many cells of a lane-kernel launch "
                         "are unfinished, they move to the cell-per-wave kernel (0: off)")
+    p.add_argument("--no-c90", action="store_true",
+                   help="skip the C90-sized leg that follows the grid workload")
     p.add_argument("--no-e2e", action="store_true",
                    help="skip the end-to-end leg (host arrays through gckpp_fullchem_step)")
     p.add_argument("--max-waves", type=int, default=0,
@@ -88,7 +90,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mech, w, ic, rc, nsample):
+def cpu_baseline(mech, w, ic, rc, nsample, reps=5):
     """The compiled reference (oracle/_ref) on an evenly strided sample of the
     workload, all host cores, Do_FullChem semantics (rates, Integrate, retry,
     clip).  Returns (baseline dict, sample indices, reference outputs)."""
@@ -101,14 +103,31 @@ def cpu_baseline(mech, w, ic, rc, nsample):
     ws = w.subset(idx)
     cores = len(os.sched_getaffinity(0))
     R.fullchem_step(ws.subset(np.arange(min(64, nsample))), ic, rc, nthreads=cores)  # warm-up
-    t0 = time.perf_counter()
-    c_ref, ist_ref, _, e_ref = R.fullchem_step(ws, ic, rc, nthreads=cores)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        c_ref, ist_ref, _, e_ref = R.fullchem_step(ws, ic, rc, nthreads=cores)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     base = {"value": round(nsample / dt, 2), "unit": "cells/s", "cores": cores, "kind": "reference",
+            "cpu_model": cpu_model(),
+            "repetitions": reps, "seconds_per_rep": [round(t, 3) for t in times],
             "sample": "%d cells spread evenly over the same %s grid (every %dth cell), reference "
                       "KPP fullchem core compiled with flang, OpenMP DYNAMIC,24 over cells, one "
-                      "chem step (%.1f s)" % (nsample, w.name, max(1, w.ncell // nsample), dt)}
+                      "chem step; median of %d repetitions (%.2f s each) after a warm-up"
+                      % (nsample, w.name, max(1, w.ncell // nsample), reps, dt)}
     return base, idx, (c_ref, ist_ref, e_ref)
+
+
+def cpu_model():
+    """The host CPU's model name (first 'model name' of /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 SIGNIFICANT = 1e5   # molec/cm3 (~4e-15 mol/mol at the surface)
@@ -132,6 +151,25 @@ def rel_errors(m, c, ref):
             "frac_within_rtol": float(np.mean(sc <= 1.0)),
             "max_scaled_err_significant": float(sc[sig].max()) if sig.any() else 0.0,
             "frac_within_rtol_significant": float(np.mean(sc[sig] <= 1.0)) if sig.any() else 1.0}
+
+
+def parity_entry(err, ncell):
+    """Pass/fail of a GPU leg against the compiled reference on the same
+    inputs: every variable species of every sampled cell within the repo
+    tolerance (scaled error <= 1)."""
+    return {"pass": bool(err["max_scaled_err"] <= 1.0), "max_scaled_err": err["max_scaled_err"],
+            "cells": int(ncell)}
+
+
+def reference_step(mech, ws, ic, rc, hstart=None):
+    """The compiled reference's chemistry step on the workload ws (per-cell
+    Hstart hstart, default ws's own); returns the concentrations."""
+    import numpy as np
+    from oracle import ref
+    if hstart is not None:
+        ws.hstart = np.ascontiguousarray(hstart, dtype=np.float64)
+    c, _, _, e = ref.RefKPP(mech).fullchem_step(ws, ic, rc, nthreads=len(os.sched_getaffinity(0)))
+    return c
 
 
 def tight_check(mech, w, idx_cells, legs, ic, rc, conc=None):
@@ -250,7 +288,7 @@ def grid_bench(args):
         st = ist.cpu().numpy()
         legs[name] = dict(ms=wall / args.steps * 1e3, cps=n / (wall / args.steps), tim=tim,
                           conc=conc.cpu().numpy(), st=st, ierr=ierr.cpu().numpy(),
-                          hnew=hnew.clone(), hos=hos)
+                          hnew=hnew.clone(), hos=hos, h0=h0.cpu().numpy())
     ctx.set_cell_order(None)
     L, P = legs["learned"], legs["plain"]
     e2e = end_to_end(args, w, ctx, ic, rc)
@@ -285,9 +323,17 @@ def grid_bench(args):
 
     base, sidx, refout = cpu_baseline(mech, w, ic, rc, args.cpu_sample)
     accuracy = {"learned_vs_plain_gpu": rel_errors(mech, L["conc"], P["conc"])}
+    parity = {}
     if refout is not None:
         accuracy["learned_vs_reference"] = rel_errors(mech, L["conc"][:, sidx], refout[0])
         accuracy["plain_vs_reference"] = rel_errors(mech, P["conc"][:, sidx], refout[0])
+        parity["C2_plain_vs_reference"] = parity_entry(accuracy["plain_vs_reference"], len(sidx))
+        if model is not None:
+            # the reference started from the learned leg's own h0 (its RCNTRL(3))
+            ref_l = reference_step(mech, w.subset(sidx), ic, rc, hstart=L["h0"][sidx])
+            accuracy["learned_vs_reference_same_h0"] = rel_errors(mech, L["conc"][:, sidx], ref_l)
+            parity["C4_learned_vs_reference_same_h0"] = parity_entry(
+                accuracy["learned_vs_reference_same_h0"], len(sidx))
     if args.tight_sample > 0 and sidx is not None:
         pick = np.linspace(0, len(sidx) - 1, args.tight_sample).astype(np.int64)
         cells = sidx[pick]
@@ -324,7 +370,7 @@ def grid_bench(args):
         "plain_gpu": {"value": round(P["cps"], 2), "unit": "cells/s", "ms_per_step": round(P["ms"], 3),
                       "solver": "one lane per cell (k_integrate), Hstart 1e-5 s"},
         "speedup_vs_plain": round(L["cps"] / P["cps"], 3),
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+        "roofline": {"bound": "fp64_valu", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
                      "traffic": traffic,
                      "kernel": "k_persist (persistent Rodas3, FP64 sparse; the algorithmic work "
@@ -342,6 +388,7 @@ def grid_bench(args):
         "warm_step": warm,
         "end_to_end": e2e,
         "accuracy": accuracy,
+        "parity": parity,
         "autotune_ms": None if tune_ms is None else [[round(float(v), 2) for v in row] for row in tune_ms],
         "cpu_baseline": base,
     }
@@ -363,15 +410,18 @@ def end_to_end(args, w, ctx, ic, rc):
     ctx.set_solver(hip.SOLVER_PERSISTENT)
     ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (w.met, w.photol, w.het)]
     tim = []
-    for k in range(2):
+    nrep = max(1, min(args.steps, 5))
+    for k in range(1 + nrep):      # the first call allocates the staging buffers: untimed
         c = np.ascontiguousarray(w.conc, dtype=np.float64).copy()
         t0 = time.perf_counter()
         hnew, ist, ierr, ok = ctx.fullchem_step_host(*ins, c, None, w.dt, ic, rc)
         tim.append(time.perf_counter() - t0)
-    wall = tim[-1]
+    tim = tim[1:]
+    wall = float(np.mean(tim))
     nbytes = sum(a.nbytes for a in ins) + 2 * c.nbytes + hnew.nbytes + ist.nbytes + ierr.nbytes
-    log("end to end (host arrays): %.1f ms" % (wall * 1e3))
+    log("end to end (host arrays): %.1f ms mean of %d calls" % (wall * 1e3, nrep))
     return {"value": round(w.ncell / wall, 2), "unit": "cells/s", "ms_per_step": round(wall * 1e3, 3),
+            "calls": nrep, "ms_min": round(min(tim) * 1e3, 3), "ms_max": round(max(tim) * 1e3, 3),
             "host_bytes_moved": int(nbytes), "failed_cells": int((ierr < 0).sum()),
             "path": "host arrays -> gckpp_fullchem_step (staged H2D, persistent solve with the "
                     "hand-off, D2H), Hstart 0, pageable memory"}
@@ -406,6 +456,19 @@ def warm_legs(args, mech, w, ctx, model, P, ic, rc):
         ctx.set_cell_order(None)
         ctx.fullchem_step(met, photol, het, conc, kpph, w.dt, ic, rc, hnew=hnew, istatus=ist, ierr=ierr)
 
+    def persistent():
+        # the persistent solver alone: the reference's Hstart = KPPHvalue, the
+        # cells ordered by it (smallest first) with the same split and hand-off
+        conc.copy_(conc0)
+        ctx.set_solver(hip.SOLVER_PERSISTENT)
+        if args.split > 0:
+            ctx.set_cell_order(hip.order_by_h0(kpph))
+            ctx.set_persistent_split(args.split)
+        else:
+            ctx.set_cell_order(None)
+        ctx.fullchem_step(met, photol, het, conc, kpph, w.dt, ic, rc, hnew=hnew, istatus=ist, ierr=ierr)
+        ctx.set_persistent_split(0)
+
     def learned():
         conc.copy_(conc0)
         ctx.set_solver(hip.SOLVER_PERSISTENT)
@@ -424,7 +487,8 @@ def warm_legs(args, mech, w, ctx, model, P, ic, rc):
 
     out = {}
     res = {}
-    for name, fn in (("plain_kpphvalue", plain), ("learned", learned)):
+    for name, fn in (("plain_kpphvalue", plain), ("persistent_kpphvalue", persistent),
+                     ("learned", learned)):
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -440,6 +504,12 @@ def warm_legs(args, mech, w, ctx, model, P, ic, rc):
     ctx.set_cell_order(None)
     out["speedup_learned_vs_plain"] = round(out["plain_kpphvalue"]["ms_per_step"] /
                                             out["learned"]["ms_per_step"], 3)
+    # what each part contributes: persistence, split and hand-off with the
+    # reference's Hstart; then the network's h0 and order on top of them
+    out["speedup_persistent_vs_plain"] = round(out["plain_kpphvalue"]["ms_per_step"] /
+                                               out["persistent_kpphvalue"]["ms_per_step"], 3)
+    out["speedup_learned_vs_persistent"] = round(out["persistent_kpphvalue"]["ms_per_step"] /
+                                                 out["learned"]["ms_per_step"], 3)
     out["attempts_saved_frac"] = round(1.0 - out["learned"]["mean_attempts"] /
                                        out["plain_kpphvalue"]["mean_attempts"], 4)
     out["learned_vs_plain"] = rel_errors(mech, res["learned"], res["plain_kpphvalue"])
@@ -648,7 +718,8 @@ def c90_bench(args):
             att.append(float(ist[2].float().mean().item()))
             acc_att += int(ist[3].sum().item())
             tot_att = int(ist[2].sum().item())
-            outs.append((conc.clone(), hnew.clone(), acc_att, tot_att))
+            outs.append((conc.clone(), hnew.clone(), acc_att, tot_att,
+                         hs[pick].cpu().numpy() if hs is not None else None))
             acc_att = 0
         ctx.set_cell_order(None)
         log("c90 %s %s step: %.1f s" % ("learned + persistent" if use_learned else "plain",
@@ -659,7 +730,7 @@ def c90_bench(args):
     for warm in (False, True):
         if warm:
             # the warm step's input: the plain leg's cold result, perturbed
-            for f, (c, hn, _, _) in zip(faces, res[("plain", False)][4]):
+            for f, (c, hn, _, _, _) in zip(faces, res[("plain", False)][4]):
                 f["warm_in"] = c.clone()
                 synth.perturb_dev(mech, f["warm_in"], f["met"], gen.manual_seed(7))
                 f["kpph"] = hn
@@ -693,9 +764,17 @@ def c90_bench(args):
     gl = np.concatenate([res[("learned", False)][4][k][0][:, pick].cpu().numpy() for k in range(6)], 1)
     gp = np.concatenate([res[("plain", False)][4][k][0][:, pick].cpu().numpy() for k in range(6)], 1)
     base, sidx, refout = cpu_baseline(mech, ws, ic, rc, ws.ncell)
+    parity = {}
     if refout is not None:
         acc["learned_vs_reference_cold"] = rel_errors(mech, gl[:, sidx], refout[0])
         acc["plain_vs_reference_cold"] = rel_errors(mech, gp[:, sidx], refout[0])
+        parity["C5_plain_vs_reference"] = parity_entry(acc["plain_vs_reference_cold"], len(sidx))
+        if model is not None:
+            hl = np.concatenate([res[("learned", False)][4][k][4] for k in range(6)])
+            ref_l = reference_step(mech, ws.subset(sidx), ic, rc, hstart=hl[sidx])
+            acc["learned_vs_reference_same_h0_cold"] = rel_errors(mech, gl[:, sidx], ref_l)
+            parity["C5_learned_vs_reference_same_h0"] = parity_entry(
+                acc["learned_vs_reference_same_h0_cold"], len(sidx))
         if args.tight_sample > 0:
             tp = np.linspace(0, len(sidx) - 1, args.tight_sample).astype(np.int64)
             cells = sidx[tp]
@@ -704,7 +783,7 @@ def c90_bench(args):
                                   "reference": refout[0][:, tp]}, ic, rc)
     # roofline of the learned leg's integrate kernels over the cold step
     flop = 0.0
-    for (_, _, nacc, ntot) in res[("learned", False)][4]:
+    for (_, _, nacc, ntot, _) in res[("learned", False)][4]:
         flop += FLOP_ACC * nacc + FLOP_REJ * (ntot - nacc)
     kms = res[("learned", False)][1]
     ho = res[("learned", False)][5]
@@ -734,10 +813,10 @@ def c90_bench(args):
                                             "%d of each chunk on the cell-per-wave kernel beside "
                                             "the lane kernel" % args.split
                                             if args.split > 0 else "one persistent kernel")},
-            "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+            "roofline": {"bound": "fp64_valu", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6), "traffic": None,
                          "kernel": "k_persist over the six chunks of the cold step"},
-            "accuracy": acc, "cpu_baseline": base}
+            "accuracy": acc, "parity": parity, "cpu_baseline": base}
 
 
 def main():
@@ -751,6 +830,16 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     out = {"grid": grid_bench, "mixed": mixed_bench, "c90": c90_bench}[args.workload](args)
+    if args.workload == "grid" and not args.no_c90:
+        # the C90-sized step (the north-star workload) beside the grid's line:
+        # one cold and one warm step of each leg, with its own roofline,
+        # accuracy, parity and CPU baseline
+        torch.cuda.empty_cache()
+        c90 = c90_bench(args)
+        out["c90"] = {k: c90[k] for k in ("value", "unit", "ms_per_step", "config", "handoff_cold",
+                                          "learned", "plain_gpu", "speedup_vs_plain", "warm_step",
+                                          "roofline", "accuracy", "cpu_baseline")}
+        out["parity"].update(c90["parity"])
     print(json.dumps(out), flush=True)
 
 

@@ -49,7 +49,8 @@ struct gckpp_ctx {
   MechInfo info;
   int64_t max_cells;
   int64_t wstride;     // cells per workspace chunk (multiple of 64)
-  double *work;        // [integrate_work_per_cell][wstride]
+  double *work;        // [integrate_work_per_cell][wstride]; persistent slots
+  int64_t work_doubles;
   double *atol, *rtol; // [NVAR] device
   double *consts;      // [3][NSPEC] device: MW, HENRY_K0, HENRY_CR
   double *step_buf;    // fullchem_step: rconst [NREACT][wstride], conc backup
@@ -113,8 +114,15 @@ static const int kDefaultPersistVariant = 4, kDefaultPersistWpb = 1;
 static int64_t cells_to_waves(int variant, int64_t ncell) {
   return variant == PERSIST_WAVE_VARIANT ? ncell : (ncell + 63) / 64;
 }
+// Fields (64 doubles) of one wave's block of the persistent workspace: a
+// lane-per-cell wave's 64 slots, or a cell-per-wave wave's scratch.
+static int64_t launch_slot_fields(int qset, int variant) {
+  return variant == PERSIST_WAVE_VARIANT ? wave_scratch_fields(qset) : persist_slot_fields(qset);
+}
 static int set_pconfig(gckpp_ctx *c, int variant, int wpb) {
   int64_t cap = c->wstride / 64 / wpb * wpb;
+  if (variant == PERSIST_WAVE_VARIANT)   // waves whose scratch fits the workspace
+    cap = c->work_doubles / (64 * wave_scratch_fields(c->mech)) / wpb * wpb;
   int64_t res = persist_resident_waves(c->mech, variant, wpb);  // PQ_* == MECH_* for one mechanism
   if (cap <= 0 || res <= 0) return -1;
   c->pvariant = variant;
@@ -149,6 +157,7 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   int64_t per_cell = integrate_work_per_cell(c->info);
   if (persist_slot_fields(mech_id) > per_cell) per_cell = persist_slot_fields(mech_id);
   size_t wbytes = sizeof(double) * (size_t)per_cell * (size_t)c->wstride;
+  c->work_doubles = per_cell * c->wstride;
   hipError_t e = hipMalloc(&c->work, wbytes);
   if (e != hipSuccess) {
     c->work = nullptr;
@@ -342,6 +351,7 @@ static int launch_persist_queue(gckpp_ctx *c, PersistArgs &pa, int64_t ncell, hi
     PersistArgs pw = pa;
     pw.qend[0] = pw.qend[1] = pw.qend[2] = k;
     pw.work = c->wwork;
+    pw.slot_fields = wave_scratch_fields(c->mech);
     pw.counter = c->wcounter;
     HIP_OK(hipEventRecord(c->ev_fork, s));
     HIP_OK(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
@@ -427,7 +437,7 @@ static int integrate_impl(gckpp_ctx *c, int64_t ncell, double tin, double tout,
     a.hlog_n = c->hlog_n;
     pa.qend[0] = pa.qend[1] = pa.qend[2] = ncell;
     pa.work = c->work;
-    pa.slot_fields = persist_slot_fields(c->mech);
+    pa.slot_fields = launch_slot_fields(c->mech, c->pvariant);
     pa.counter = c->counter;
     return launch_persist_queue(c, pa, ncell, s, "gckpp_integrate");
   }
@@ -618,7 +628,7 @@ static int step_impl(gckpp_ctx *c, int64_t ncell, double dt, const int32_t icntr
       pa.q[0].hlog_n = c->hlog_n;
       pa.qend[0] = pa.qend[1] = pa.qend[2] = ncell;
       pa.work = c->work;
-      pa.slot_fields = persist_slot_fields(c->mech);
+      pa.slot_fields = launch_slot_fields(c->mech, c->pvariant);
       pa.counter = c->counter;
       int rcl = launch_persist_queue(c, pa, ncell, s, "gckpp_fullchem_step");
       if (rcl) return rcl;
@@ -1098,7 +1108,7 @@ static int batch_enqueue(gckpp_batch *b, const gckpp_part parts[3], double dt, h
   }
   if (end == 0) return 0;
   pa.work = b->work;
-  pa.slot_fields = b->slot_fields;
+  pa.slot_fields = launch_slot_fields(PQ_MIXED, b->variant);
   pa.counter = b->counter;
   const bool hand = b->ho_cap > 0 && b->ho_rec && b->variant != PERSIST_WAVE_VARIANT;
   if (hand) {
@@ -1146,12 +1156,11 @@ extern "C" int gckpp_batch_set_handoff(gckpp_batch *b, int64_t max_cells) {
     // before the old list is released)
     if (batch_drop_graph(b) != 0) return fail("gckpp_batch_set_handoff: releasing the graph failed");
     HIP_OK(hipDeviceSynchronize());
-    int64_t nvar = 0, hwf = 0;
+    int64_t nvar = 0;
+    const int64_t hwf = wave_scratch_fields(PQ_MIXED);
     for (int q = 0; q < 3; ++q) {
       const MechInfo mi = mech_info(q);  // queue q holds mechanism q
       nvar = mi.nvar > nvar ? mi.nvar : nvar;
-      const int64_t f = (mi.nreact + 5 * (int64_t)mi.nvar + 63) / 64;
-      hwf = f > hwf ? f : hwf;
     }
     for (void *p : {(void *)b->ho_cell, (void *)b->ho_rec})
       if (p) (void)hipFree(p);
@@ -1439,7 +1448,7 @@ extern "C" int gckpp_set_persistent_split(gckpp_ctx *c, int64_t nfirst) {
   if (nfirst > 0 && !c->wwork) {
     const int64_t res = persist_resident_waves(c->mech, PERSIST_WAVE_VARIANT, 1);
     if (res <= 0) return fail("gckpp_set_persistent_split: cannot size the cell-per-wave launch");
-    const size_t wb = sizeof(double) * (size_t)res * 64 * (size_t)persist_slot_fields(c->mech);
+    const size_t wb = sizeof(double) * (size_t)res * 64 * (size_t)wave_scratch_fields(c->mech);
     hipError_t e = hipMalloc(&c->wwork, wb);
     if (e == hipSuccess) e = hipMalloc(&c->wcounter, sizeof(unsigned int));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking);
@@ -1489,7 +1498,7 @@ extern "C" int gckpp_set_persistent_handoff(gckpp_ctx *c, int64_t max_cells) {
     if (e == hipSuccess && !c->hwork) {
       const int64_t res = persist_resident_waves(c->mech, PERSIST_WAVE_VARIANT, 1);
       if (res <= 0) e = hipErrorInvalidConfiguration;
-      c->hw_fields = (mi.nreact + 5 * (int64_t)mi.nvar + 63) / 64;
+      c->hw_fields = wave_scratch_fields(c->mech);
       c->hwaves = res;
       if (e == hipSuccess)
         e = hipMalloc(&c->hwork, sizeof(double) * 64 * (size_t)c->hw_fields * (size_t)res);

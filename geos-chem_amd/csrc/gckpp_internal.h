@@ -123,6 +123,7 @@ int persist_variant_info(int variant, int *U, int *W);
 int persist_variant_cache(int variant);  // operand-cache slots per lane (0: none)
 int persist_variant_seg(int variant);  // 0 lane per cell, 1 with flat segments, 2 wave per cell
 const void *wave_kernel(int qset);     // the cell-per-wave kernel (64 threads per workgroup)
+int64_t wave_scratch_fields(int qset);  // its global scratch per wave, fields of 64 doubles
 int64_t persist_slot_fields(int qset);                   // doubles per lane slot
 int64_t persist_resident_waves(int qset, int variant, int wpb);  // 0 on error
 int launch_persist(int qset, int variant, int wpb, const PersistArgs &pa, int64_t nwaves,

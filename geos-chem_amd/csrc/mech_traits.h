@@ -11,12 +11,13 @@
 namespace gckpp {
 
 // One program of the cell-per-wave solver (see mechgen/wave_sched.py):
-// levels -> rounds -> steps, per round and lane a head word, per step and
-// lane a term word.
+// per block of four steps and lane a uint4 of term words, per block four
+// flag bits (eight blocks a word), per round and lane a head word; nb
+// blocks, a multiple of eight.
 struct WProg {
-  const int *lev, *rs;
-  const unsigned *head, *term;
-  int nlev;
+  const uint4 *term;
+  const unsigned *flag, *head;
+  int nb;
 };
 enum { WP_FV = 0, WP_JV = 1, WP_LU = 2, WP_SF = 3, WP_SB = 4 };
 // Mechanism-generic device pieces.  M is one of the mech_<name> namespaces,
@@ -76,28 +77,28 @@ enum { WP_FV = 0, WP_JV = 1, WP_LU = 2, WP_SF = 3, WP_SB = 4 };
     static constexpr int NSF_BATCH = mech_##NS::NSF_BATCH;                     \
     static constexpr int NSB_BATCH = mech_##NS::NSB_BATCH;                     \
     static constexpr int WV_ZERO = mech_##NS::WV_ZERO, WV_X0 = mech_##NS::WV_X0; \
-    static constexpr int WV_Y0 = mech_##NS::WV_Y0, WV_YN0 = mech_##NS::WV_YN0; \
-    static constexpr int WV_AB0 = mech_##NS::WV_AB0;                           \
-    static constexpr int WV_COEF0 = mech_##NS::WV_COEF0;                       \
-    static constexpr int WV_NCOEF = mech_##NS::WV_NCOEF, WV_SH = mech_##NS::WV_SH; \
-    static __device__ double wv_coef(int i) { return mech_##NS::wv_coef[i]; }  \
+    static constexpr int WV_Y0 = mech_##NS::WV_Y0, WV_SH = mech_##NS::WV_SH;   \
+    static constexpr int WV_G_AB = mech_##NS::WV_G_AB, WV_NAB = mech_##NS::WV_NAB; \
+    static constexpr int WV_G_F0 = mech_##NS::WV_G_F0, WV_G_K = mech_##NS::WV_G_K; \
+    static constexpr int WV_G_SIZE = mech_##NS::WV_G_SIZE;                     \
+    static __device__ const double *wv_coef() { return mech_##NS::wv_coef; }   \
     static __device__ WProg wprog(int p) {                                     \
       switch (p) {                                                             \
-        case WP_FV: return WProg{mech_##NS::wv_fv_lev, mech_##NS::wv_fv_rs,    \
-                                 mech_##NS::wv_fv_head, mech_##NS::wv_fv_term, \
-                                 mech_##NS::WV_FV_NLEV};                       \
-        case WP_JV: return WProg{mech_##NS::wv_jv_lev, mech_##NS::wv_jv_rs,    \
-                                 mech_##NS::wv_jv_head, mech_##NS::wv_jv_term, \
-                                 mech_##NS::WV_JV_NLEV};                       \
-        case WP_LU: return WProg{mech_##NS::wv_lu_lev, mech_##NS::wv_lu_rs,    \
-                                 mech_##NS::wv_lu_head, mech_##NS::wv_lu_term, \
-                                 mech_##NS::WV_LU_NLEV};                       \
-        case WP_SF: return WProg{mech_##NS::wv_sf_lev, mech_##NS::wv_sf_rs,    \
-                                 mech_##NS::wv_sf_head, mech_##NS::wv_sf_term, \
-                                 mech_##NS::WV_SF_NLEV};                       \
-        default: return WProg{mech_##NS::wv_sb_lev, mech_##NS::wv_sb_rs,       \
-                              mech_##NS::wv_sb_head, mech_##NS::wv_sb_term,    \
-                              mech_##NS::WV_SB_NLEV};                          \
+        case WP_FV: return WProg{(const uint4 *)mech_##NS::wv_fv_term,         \
+                                 mech_##NS::wv_fv_flag, mech_##NS::wv_fv_head, \
+                                 mech_##NS::WV_FV_NB};                         \
+        case WP_JV: return WProg{(const uint4 *)mech_##NS::wv_jv_term,         \
+                                 mech_##NS::wv_jv_flag, mech_##NS::wv_jv_head, \
+                                 mech_##NS::WV_JV_NB};                         \
+        case WP_LU: return WProg{(const uint4 *)mech_##NS::wv_lu_term,         \
+                                 mech_##NS::wv_lu_flag, mech_##NS::wv_lu_head, \
+                                 mech_##NS::WV_LU_NB};                         \
+        case WP_SF: return WProg{(const uint4 *)mech_##NS::wv_sf_term,         \
+                                 mech_##NS::wv_sf_flag, mech_##NS::wv_sf_head, \
+                                 mech_##NS::WV_SF_NB};                         \
+        default: return WProg{(const uint4 *)mech_##NS::wv_sb_term,            \
+                              mech_##NS::wv_sb_flag, mech_##NS::wv_sb_head,    \
+                              mech_##NS::WV_SB_NB};                            \
       }                                                                        \
     }                                                                          \
   };

@@ -1,13 +1,15 @@
 // Rodas3 solve with one wavefront per cell.  The lane-per-cell solvers
 // (rosenbrock.hip) keep each cell's state in HBM and stream ~1.3 MB per
-// Rodas3 attempt; here the cell's LU, solve vector, state and reaction
-// scratch live in LDS (fullchem: 71.6 KB, two cells per CU) and the 64 lanes
-// share its sums.  Every sum -- a Vdot row, a Jacobian entry, an LU position
-// (row-Crout form), a substitution row -- is computed by one lane, term by
-// term in the same order and with the same operations as the lane-per-cell
-// solvers, so the results are bit-identical to theirs.  The mechanism's wave
-// programs (gen/mech_<name>.h, built by mechgen/wave_sched.py) say which lane
-// computes which chunk of which sum at which dependency level.
+// Rodas3 attempt; here the cell's LU, solve vector and state live in LDS
+// (fullchem: 51 KB, three cells per CU) and the 64 lanes share its sums.
+// The reaction products, Jacobian partials, rate constants, Fcn0 and the
+// stage vectors sit in the wave's global scratch (about 37 KB, L2-resident).
+// Every sum -- a Vdot row, a Jacobian entry, an LU position (row-Crout
+// form), a substitution row -- is computed by one lane, term by term in the
+// same order and with the same operations as the lane-per-cell solvers, so
+// the results are bit-identical to theirs.  The mechanism's wave programs
+// (gen/mech_<name>.h, built by mechgen/wave_sched.py) say which lane
+// computes which chunk of which sum in which round.
 //
 // Algorithm: that of lane_attempt in rosenbrock.hip (ros_Integrator,
 // KPP/fullchem/gckpp_Integrator.F90:578-786, Rodas3 :2239-2303).
@@ -25,6 +27,7 @@ namespace gckpp {
 namespace {
 
 constexpr unsigned kIdle = 0xFFFFu;
+constexpr int kPF = 8;  // blocks of term words loaded ahead (wave_sched.PF)
 
 // Orders the wave's LDS accesses across lanes: LDS instructions of one
 // wavefront execute in program order, so it suffices that the compiler moves
@@ -35,80 +38,119 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Global scratch written by some lanes and read by others (the rate
-// constants): wait for the wave's stores first.
+// Global scratch written by some lanes and read by others (rate constants,
+// reaction products, partials, stage vectors): wait for the wave's memory
+// operations first.
 __device__ __forceinline__ void wave_sync_global() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Runs one wave program.  MODE 0 (Vdot, Jacobian): a chunk's sum starts at
-// +0.0 and a step is acc = fma(b, a, acc); MODE 1 (LU, substitution): it
-// starts at the destination slot and a step is acc = fma(-a, b, acc).
-// store(dest, hi, acc) writes a chunk's result.  Rounds go in one flat loop
-// over the program's levels; the head word and first block of 8 term words
-// of the next round are loaded while the current round runs (the arrays end
-// with an idle round and a block of pads, so these loads stay inside them).
-template <int MODE, int ZERO, int COEF0, class Store>
-__device__ __forceinline__ void run_prog(double *sh, const WProg &p, int lane, Store store) {
-  // pad step: operand a = ZERO (+0.0), b = ZERO or the -0.0 coefficient
-  constexpr unsigned PAD = (unsigned)ZERO | ((unsigned)(MODE == 0 ? COEF0 : ZERO) << 16);
-  const int R = p.lev[p.nlev];
-  int l = 0, lend = p.nlev > 0 ? p.lev[1] : 0;
-  unsigned h = p.head[lane];
-  unsigned w[8];
-  {
-    const unsigned *tp = p.term + (size_t)p.rs[0] * 64 + lane;
+__device__ __forceinline__ double ld(const char *base, unsigned off) {
+  return *(const double *)(base + off);
+}
+__device__ __forceinline__ void ld_st(double *sh, unsigned off, double v) {
+  *(double *)((char *)sh + off) = v;
+}
+
+// Runs one wave program.  MODE 0 (Vdot, Jacobian): operand a from the
+// reaction scratch gab, b from the coefficient table coef (byte offsets);
+// a chunk's sum starts at +0.0 and a step is acc = fma(b, a, acc).  MODE 1
+// (LU, substitution): both operands in LDS (shb); the sum starts at the
+// destination slot and a step is acc = fma(-a, b, acc); DIV: a round's
+// result is divided by its divisor slot when it names one.  store(dest,
+// hi, acc) writes a round's result (byte offsets).
+//
+// One flat loop over blocks of four steps.  The term words of the next kPF
+// blocks are in flight while a block runs, the head words of the next
+// three rounds likewise.  A block's flag bits (uniform) say where rounds
+// start and end and where a level ends.  The operands of the next block --
+// and, when it starts a round, its starting value and divisor -- are
+// requested before the current block's FMAs unless the current block ends
+// a level: within a level no chunk reads what another chunk of the level
+// stores, so those reads may pass the level's stores; the next level's
+// reads are issued after the ordering point.
+template <int MODE, bool DIV, int ZB, class Store>
+__device__ __forceinline__ void run_prog(char *shb, const WProg &p, int lane, const char *gab,
+                                         const char *coef, Store store) {
+  const uint4 *tp = p.term + lane;
+  uint4 ring[kPF];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) w[u] = tp[u * 64];
-  }
-#pragma unroll 1
-  for (int r = 0; r < R; ++r) {
-    const int s0 = p.rs[r], n = p.rs[r + 1] - s0;
-    // next round's head and first block
-    const unsigned hn = p.head[(r + 1) * 64 + lane];
-    unsigned wn[8];
-    {
-      const unsigned *tq = p.term + (size_t)(s0 + n) * 64 + lane;
+  for (int i = 0; i < kPF; ++i) ring[i] = tp[i * 64];
+  const unsigned *hp = p.head + lane;
+  unsigned h0 = hp[0], h1 = hp[64], h2 = hp[128];
+  double opa[2][4], opb[2][4], init[2], divv[2], dv = 1.0;
+  // requests block operands (words w), and a round start's value/divisor
+  auto fetch = [&](int slot, const uint4 &w, unsigned f, unsigned h) {
+    const unsigned ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-      for (int u = 0; u < 8; ++u) wn[u] = tq[u * 64];
-    }
-    const unsigned dest = h & 0xFFFFu;
-    const bool act = dest != kIdle;
-    double acc = 0.0;
-    if constexpr (MODE == 1) acc = sh[act ? dest : (unsigned)ZERO];
-    const unsigned *tp = p.term + (size_t)s0 * 64 + lane;
-#pragma unroll 1
-    for (int s = 0; s < n; s += 8) {
-      if (s > 0) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) w[u] = tp[(s + u) * 64];
+    for (int u = 0; u < 4; ++u) {
+      const unsigned ao = ww[u] & 0xFFFFu, bo = ww[u] >> 16;
+      if constexpr (MODE == 1) {
+        opa[slot][u] = ld(shb, ao);
+        opb[slot][u] = ld(shb, bo);
+      } else {
+        opa[slot][u] = ld(gab, ao);
+        opb[slot][u] = ld(coef, bo);
       }
-      // past the round's last step the words belong to the next round or
-      // the final pads: use the pad (product -0.0)
+    }
+    if constexpr (MODE == 1) {
+      if (f & 1u) {
+        const unsigned d = h & 0xFFFFu;
+        init[slot] = ld(shb, d != kIdle ? d : (unsigned)ZB);
+        if constexpr (DIV) {
+          const unsigned v = h >> 16;
+          divv[slot] = ld(shb, v != kIdle ? v : (unsigned)ZB);
+        }
+      }
+    }
+  };
+  int r = 0;
+  double acc = 0.0;
+  unsigned fl = p.flag[0];
+  fetch(0, ring[0], fl & 15u, h0);
+#pragma unroll 1
+  for (int k0 = 0; k0 < p.nb; k0 += kPF) {
+    const unsigned fln = p.flag[(k0 + kPF) >> 3];
 #pragma unroll
-      for (int u = 1; u < 8; ++u)
-        if (s + u >= n) w[u] = PAD;
-      double a[8], b[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        a[u] = sh[w[u] & 0xFFFFu];
-        b[u] = sh[w[u] >> 16];
+    for (int i = 0; i < kPF; ++i) {
+      const int cur = i & 1, nxt = cur ^ 1;
+      const unsigned f = (fl >> (4 * i)) & 15u;
+      const unsigned fn = i + 1 < kPF ? (fl >> (4 * (i + 1))) & 15u : fln & 15u;
+      const unsigned hn = (f & 2u) ? h1 : h0;  // head of the next block's round
+      const uint4 wn = ring[(i + 1) % kPF];
+      if (!(f & 4u)) fetch(nxt, wn, fn, hn);
+      ring[i] = tp[(k0 + i + kPF) * 64];
+      if (f & 1u) {
+        acc = MODE == 1 ? init[cur] : 0.0;
+        if constexpr (DIV) dv = divv[cur];
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc = MODE == 0 ? fma(b[u], a[u], acc) : fma(-a[u], b[u], acc);
+      for (int u = 0; u < 4; ++u)
+        acc = MODE == 0 ? fma(opb[cur][u], opa[cur][u], acc) : fma(-opa[cur][u], opb[cur][u], acc);
+      if (f & 2u) {
+        const unsigned dest = h0 & 0xFFFFu;
+        if (dest != kIdle) {
+          double v = acc;
+          if constexpr (DIV) {
+            if ((h0 >> 16) != kIdle) v = v / dv;
+          }
+          store(dest, h0 >> 16, v);
+        }
+        h0 = h1;
+        h1 = h2;
+        h2 = hp[(r + 3) * 64];
+        ++r;
+      }
+      if (f & 4u) {
+        wave_sync();
+        fetch(nxt, wn, fn, hn);
+      }
     }
-    if (act) store(dest, h >> 16, acc);
-    if (r + 1 == lend) {  // end of a level: its results become the next level's operands
-      wave_sync();
-      ++l;
-      if (l < p.nlev) lend = p.lev[l + 1];
-    }
-    h = hn;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) w[u] = wn[u];
+    fl = fln;
   }
+  wave_sync();
 }
 
 // Per-cell integrator state, the same on every lane.
@@ -120,16 +162,18 @@ struct WaveState {
 };
 
 // The cell a wave works on: LDS block sh (layout WV_* of the mechanism) and
-// the wave's global scratch g: rate constants [NREACT], Fcn0 [NVAR], the
-// stage vectors K [4][NVAR].
+// the wave's global scratch g (WV_G_*): rate constants [NREACT], reaction
+// products or partials [NAB] and a +0.0 slot, Fcn0 [NVAR], the stage
+// vectors K [4][NVAR].
 template <class M>
 struct WaveCell {
   double *sh;
   double *g;
   int lane;
   __device__ double *rct() const { return g; }
-  __device__ double *F0() const { return g + M::NREACT; }
-  __device__ double *K(int s) const { return g + M::NREACT + (1 + s) * M::NVAR; }
+  __device__ double *ab() const { return g + M::WV_G_AB; }
+  __device__ double *F0() const { return g + M::WV_G_F0; }
+  __device__ double *K(int s) const { return g + M::WV_G_K + s * M::NVAR; }
 };
 
 // Fun: A(r) = RCT(r)*prod conc(reactants) into the scratch, then Vdot into
@@ -138,6 +182,8 @@ template <class M>
 __device__ void wave_fun(const WaveCell<M> &c, int in0) {
   double *sh = c.sh;
   const double *rct = c.rct();
+  double *ab = c.ab();
+  wave_sync_global();
 #pragma unroll 1
   for (int r = c.lane; r < M::NREACT; r += 64) {
     double v = rct[r];
@@ -147,19 +193,21 @@ __device__ void wave_fun(const WaveCell<M> &c, int in0) {
       if (s < 0) break;
       v *= s < M::NVAR ? sh[in0 + s] : sh[M::WV_Y0 + s];
     }
-    sh[M::WV_AB0 + r] = v;
+    ab[r] = v;
   }
-  wave_sync();
-  run_prog<0, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_FV), c.lane,
-                          [&](unsigned d, unsigned, double acc) { sh[d] = acc; });
+  wave_sync_global();
+  run_prog<0, false, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_FV), c.lane, (const char *)ab,
+                              (const char *)M::wv_coef(),
+                              [&](unsigned d, unsigned, double acc) { ld_st(sh, d, acc); });
 }
-
 // Ghimj = sign*JVS + diag on the diagonal into the LU block (partials from Y
 // into the scratch first); returns IER of KppDecomp's pivot check.
 template <class M>
 __device__ int wave_jac(const WaveCell<M> &c, double sign, double diag) {
   double *sh = c.sh;
   const double *rct = c.rct();
+  double *ab = c.ab();
+  wave_sync_global();
 #pragma unroll 1
   for (int p = c.lane; p < M::NPART; p += 64) {
     double v = M::pt_m(p) * rct[M::pt_r(p)];
@@ -169,14 +217,16 @@ __device__ int wave_jac(const WaveCell<M> &c, double sign, double diag) {
       if (s < 0) break;
       v *= sh[M::WV_Y0 + s];
     }
-    sh[M::WV_AB0 + p] = v;
+    ab[p] = v;
   }
-  wave_sync();
-  run_prog<0, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_JV), c.lane, [&](unsigned d, unsigned isdiag, double acc) {
-    double v = sign * acc;
-    if (isdiag) v = v + diag;
-    sh[d] = v;
-  });
+  wave_sync_global();
+  run_prog<0, false, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_JV), c.lane, (const char *)ab,
+                              (const char *)M::wv_coef(),
+                              [&](unsigned d, unsigned isdiag, double acc) {
+                                double v = sign * acc;
+                                if (isdiag) v = v + diag;
+                                ld_st(sh, d, v);
+                              });
   // the first row (1-based) whose diagonal was below TINY before its
   // elimination
 #pragma unroll 1
@@ -192,21 +242,26 @@ __device__ int wave_jac(const WaveCell<M> &c, double sign, double diag) {
 template <class M>
 __device__ void wave_decomp(const WaveCell<M> &c) {
   double *sh = c.sh;
-  run_prog<1, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_LU), c.lane, [&](unsigned d, unsigned dv, double acc) {
-    if (dv) acc = acc / sh[dv - 1];
-    sh[d] = acc;
-  });
+  run_prog<1, true, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_LU), c.lane, nullptr, nullptr,
+                                    [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
 }
 
 template <class M>
 __device__ void wave_solve(const WaveCell<M> &c) {
   double *sh = c.sh;
-  run_prog<1, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_SF), c.lane,
-                          [&](unsigned d, unsigned, double acc) { sh[d] = acc; });
-  run_prog<1, M::WV_ZERO, M::WV_COEF0>(sh, M::wprog(WP_SB), c.lane, [&](unsigned d, unsigned dv, double acc) {
-    if (dv) acc = acc / sh[dv - 1];
-    sh[d] = acc;
-  });
+  run_prog<1, false, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_SF), c.lane, nullptr, nullptr,
+                                     [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
+  run_prog<1, true, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_SB), c.lane, nullptr, nullptr,
+                                    [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
+}
+
+// The cell's constant slots: +0.0 in LDS and in the scratch's pad slot.
+template <class M>
+__device__ __forceinline__ void wave_zero_slots(const WaveCell<M> &c) {
+  if (c.lane == 0) {
+    c.sh[M::WV_ZERO] = 0.0;
+    c.ab()[M::WV_NAB] = 0.0;
+  }
 }
 
 template <class M>
@@ -214,13 +269,12 @@ __device__ void wave_begin(const IntegrateArgs &a, const WaveCell<M> &c, int64_t
                            WaveState &S) {
   double *sh = c.sh;
   double *rct = c.rct();
+  wave_sync_global();
 #pragma unroll 1
   for (int s = c.lane; s < M::NSPEC; s += 64) sh[M::WV_Y0 + s] = a.conc[(int64_t)s * a.cstride + cell];
 #pragma unroll 1
   for (int r = c.lane; r < M::NREACT; r += 64) rct[r] = a.rconst[(int64_t)r * a.rstride + cell];
-#pragma unroll 1
-  for (int i = c.lane; i < M::WV_NCOEF; i += 64) sh[M::WV_COEF0 + i] = M::wv_coef(i);
-  if (c.lane == 0) sh[M::WV_ZERO] = 0.0;
+  wave_zero_slots(c);
   wave_sync_global();
   const RosParams &P = a.prm;
   const double hs = (a.hstart && !cold) ? a.hstart[cell] : 0.0;
@@ -244,6 +298,7 @@ template <class M>
 __device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64_t cell,
                              WaveState &S) {
   constexpr int N = M::NVAR;
+  constexpr int KS = (N + 63) / 64;  // species per lane: i = lane + 64 k
   double *sh = c.sh;
   const int lane = c.lane;
   const RosParams &P = a.prm;
@@ -281,10 +336,12 @@ __device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64
   const double H = S.H;
 #pragma unroll 1
   for (int s = 0; s < 4; ++s) {
+    wave_sync_global();
     if (s == 0 || !kRosNewF[s]) {
 #pragma unroll 1
       for (int i = lane; i < N; i += 64) sh[M::WV_X0 + i] = F0[i];
     } else {
+      // the stage point Ynew in X, then Fun(Ynew) into X
 #pragma unroll 1
       for (int i = lane; i < N; i += 64) {
         double v = sh[M::WV_Y0 + i];
@@ -292,10 +349,10 @@ __device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64
           const double aj = kRosA[s * (s - 1) / 2 + j];
           if (aj != 0.0) v = fma(aj, c.K(j)[i], v);
         }
-        sh[M::WV_YN0 + i] = v;
+        sh[M::WV_X0 + i] = v;
       }
       wave_sync();
-      wave_fun<M>(c, M::WV_YN0);
+      wave_fun<M>(c, M::WV_X0);
       S.st[0] += 1;
     }
     for (int j = 0; j < s; ++j) {
@@ -313,21 +370,27 @@ __device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64
     for (int i = lane; i < N; i += 64) Ks[i] = sh[M::WV_X0 + i];
     S.st[6] += 1;
   }
-  // Ynew and the scaled error terms per species, then the error sum in
-  // species order (every lane sums the same terms)
-#pragma unroll 1
-  for (int i = lane; i < N; i += 64) {
-    const double y0 = sh[M::WV_Y0 + i];
-    double v = y0;
-    for (int j = 0; j < 4; ++j)
-      if (kRosM[j] != 0.0) v = fma(kRosM[j], c.K(j)[i], v);
-    double ye = 0.0;
-    for (int j = 0; j < 4; ++j)
-      if (kRosE[j] != 0.0) ye = fma(kRosE[j], c.K(j)[i], ye);
-    const double ymax = max_f(fabs(y0), fabs(v));
-    const double sc = P.vectol ? a.atol[i] + a.rtol[i] * ymax : a.atol[0] + a.rtol[0] * ymax;
-    sh[M::WV_YN0 + i] = v;
-    sh[M::WV_AB0 + i] = ye / sc;
+  wave_sync_global();
+  // Ynew (kept per lane) and the scaled error terms per species in X, then
+  // the error sum in species order (every lane sums the same terms)
+  double yn[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int i = lane + 64 * k;
+    yn[k] = 0.0;
+    if (i < N) {
+      const double y0 = sh[M::WV_Y0 + i];
+      double v = y0;
+      for (int j = 0; j < 4; ++j)
+        if (kRosM[j] != 0.0) v = fma(kRosM[j], c.K(j)[i], v);
+      double ye = 0.0;
+      for (int j = 0; j < 4; ++j)
+        if (kRosE[j] != 0.0) ye = fma(kRosE[j], c.K(j)[i], ye);
+      const double ymax = max_f(fabs(y0), fabs(v));
+      const double sc = P.vectol ? a.atol[i] + a.rtol[i] * ymax : a.atol[0] + a.rtol[0] * ymax;
+      yn[k] = v;
+      sh[M::WV_X0 + i] = ye / sc;
+    }
   }
   wave_sync();
   double err = 0.0;
@@ -337,13 +400,13 @@ __device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64
     for (; i + 8 <= N; i += 8) {
       double q[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) q[u] = sh[M::WV_AB0 + i + u];
+      for (int u = 0; u < 8; ++u) q[u] = sh[M::WV_X0 + i + u];
 #pragma unroll
       for (int u = 0; u < 8; ++u) err = fma(q[u], q[u], err);
     }
 #pragma unroll 1
     for (; i < N; ++i) {
-      const double q = sh[M::WV_AB0 + i];
+      const double q = sh[M::WV_X0 + i];
       err = fma(q, q, err);
     }
   }
@@ -357,10 +420,10 @@ __device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64
   S.st[2] += 1;
   if (err <= 1.0 || H <= P.Hmin) {
     S.st[3] += 1;
-#pragma unroll 1
-    for (int i = lane; i < N; i += 64) {
-      const double v = sh[M::WV_YN0 + i];
-      sh[M::WV_Y0 + i] = P.clip_negative ? max_f(v, 0.0) : v;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int i = lane + 64 * k;
+      if (i < N) sh[M::WV_Y0 + i] = P.clip_negative ? max_f(yn[k], 0.0) : yn[k];
     }
     S.T = S.T + dir * H;
     S.Hnew = max_f(P.Hmin, min_f(S.Hnew, P.Hmax));
@@ -418,14 +481,13 @@ __device__ void wave_resume(const IntegrateArgs &a, const Handoff &ho, const Wav
   double *rct = c.rct();
   const double *r = ho.rec + idx;
   const int64_t cap = ho.cap;
+  wave_sync_global();
 #pragma unroll 1
   for (int s = c.lane; s < M::NSPEC; s += 64)
     sh[M::WV_Y0 + s] = s < M::NVAR ? r[(HO_FIELDS + s) * cap] : a.conc[(int64_t)s * a.cstride + cell];
 #pragma unroll 1
   for (int k = c.lane; k < M::NREACT; k += 64) rct[k] = a.rconst[(int64_t)k * a.rstride + cell];
-#pragma unroll 1
-  for (int i = c.lane; i < M::WV_NCOEF; i += 64) sh[M::WV_COEF0 + i] = M::wv_coef(i);
-  if (c.lane == 0) sh[M::WV_ZERO] = 0.0;
+  wave_zero_slots(c);
   wave_sync_global();
   S.T = r[0];
   S.H = r[cap];
@@ -578,6 +640,21 @@ __global__ __launch_bounds__(64) void k_wave(PersistArgs pa) {
 }
 
 }  // namespace
+
+// Global scratch of one cell-per-wave wave, in fields of 64 doubles (the
+// unit of PersistArgs::slot_fields): the largest WV_G_SIZE of the set.
+int64_t wave_scratch_fields(int qset) {
+  int64_t g = 0;
+  switch (qset) {
+    case PQ_FULLCHEM: g = Traits_fullchem::WV_G_SIZE; break;
+    case PQ_HG: g = Traits_Hg::WV_G_SIZE; break;
+    case PQ_CARBON: g = Traits_carbon::WV_G_SIZE; break;
+    case PQ_MIXED:
+      g = cmax(Traits_fullchem::WV_G_SIZE, cmax(Traits_Hg::WV_G_SIZE, Traits_carbon::WV_G_SIZE));
+      break;
+  }
+  return (g + 63) / 64;
+}
 
 // The cell-per-wave kernel for a queue set (PQ_*), launched with 64 threads
 // per workgroup.

@@ -18,8 +18,8 @@ wavefront-uniform index, so the compiler reads it through the scalar cache.
                         lp in [le_p0, le_p1)
   cu_*, sf_*, sb_*      the same sums as flat segment lists (see build_tables)
                         for the persistent solver's batched execution
-  wv_*, WV_*            the cell-per-wave solver's programs and LDS layout
-                        (wave_sched.py)
+  wv_*, WV_*            the cell-per-wave solver's programs, LDS and scratch
+                        layouts (wave_sched.py)
   lc<S>_lu, lc<S>_sv    the persistent solver's operand-cache schedules with
                         S slots per lane (lane_cache.py)
 Run:  python emit_tables.py <mech> [<mech> ...]
@@ -36,7 +36,7 @@ HERE = Path(__file__).resolve().parent
 RX_MAX = 3
 
 
-def arr(ctype, name, vals, per_line=16, fmt=str):
+def arr(ctype, name, vals, per_line=16, fmt=str, align=0):
     vals = list(vals)
     if not vals:
         vals = [0]
@@ -44,7 +44,9 @@ def arr(ctype, name, vals, per_line=16, fmt=str):
     for i in range(0, len(vals), per_line):
         lines.append("  " + ", ".join(fmt(v) for v in vals[i:i + per_line]) + ",")
     # internal linkage: the header is included by more than one translation unit
-    return "static __constant__ %s %s[%d] = {\n%s\n};\n" % (ctype, name, len(vals), "\n".join(lines))
+    al = " __attribute__((aligned(%d)))" % align if align else ""
+    return "static __constant__ %s %s[%d]%s = {\n%s\n};\n" % (ctype, name, len(vals), al,
+                                                               "\n".join(lines))
 
 
 def dfmt(v):
@@ -225,24 +227,30 @@ def emit(name):
         out.append(arr("double", k, t[k], per_line=6, fmt=dfmt))
     for k, key in (("lu_crow", "lu_crow"), ("lu_diag", "lu_diag"), ("lu_icol", "lu_icol")):
         out.append(arr("int", k, m[key][: (m["nvar"] + 1) if k != "lu_icol" else None]))
-    # cell-per-wave solver: LDS layout and programs
+    # cell-per-wave solver: LDS layout, global scratch layout and programs
     prog = wave_sched.build_wave_program(m, t)
     L = prog["layout"]
-    out.append("constexpr int WV_ZERO = %d, WV_X0 = %d, WV_Y0 = %d, WV_YN0 = %d, WV_AB0 = %d;"
-               % (L["ZERO"], L["X0"], L["Y0"], L["YN0"], L["AB0"]))
-    out.append("constexpr int WV_COEF0 = %d, WV_NCOEF = %d, WV_SH = %d;"
-               % (L["COEF0"], len(prog["coef"]), L["SH_SIZE"]))
-    nlev = {P: (len(prog[P][0][0]) - 1 if P in prog else 0) for P in ("fv", "jv", "lu", "sf", "sb")}
-    out.append("constexpr int WV_FV_NLEV = %d, WV_JV_NLEV = %d, WV_LU_NLEV = %d, WV_SF_NLEV = %d, "
-               "WV_SB_NLEV = %d;" % tuple(nlev[P] for P in ("fv", "jv", "lu", "sf", "sb")))
+    out.append("constexpr int WV_ZERO = %d, WV_X0 = %d, WV_Y0 = %d, WV_SH = %d;"
+               % (L["ZERO"], L["X0"], L["Y0"], L["SH_SIZE"]))
+    out.append("constexpr int WV_G_AB = %d, WV_NAB = %d, WV_G_F0 = %d, WV_G_K = %d, WV_G_SIZE = %d, "
+               "WV_NCOEF = %d;" % (L["G_AB"], L["NAB"], L["G_F0"], L["G_K"], L["G_SIZE"],
+                                   len(prog["coef"])))
+    arrays, consts = wave_sched.emit_arrays(prog)
     have = {P for P in ("fv", "jv", "lu", "sf", "sb") if P in prog}
-    for aname, ctype, vals in wave_sched.emit_arrays(prog):
+    cmap = dict(consts)
+    for P in ("fv", "jv", "lu", "sf", "sb"):
+        out.append("constexpr int WV_%s_NB = %d, WV_%s_NR = %d;"
+                   % (P.upper(), cmap.get("WV_%s_NB" % P.upper(), 0), P.upper(),
+                      cmap.get("WV_%s_NR" % P.upper(), 0)))
+    for aname, ctype, vals in arrays:   # term words are read as 16-byte blocks
         out.append(arr(ctype, aname, vals, per_line=6 if ctype == "double" else 16,
-                       fmt=dfmt if ctype == "double" else str))
+                       fmt=dfmt if ctype == "double" else str,
+                       align=16 if aname.endswith("_term") else 0))
     for P in ("fv", "jv", "lu", "sf", "sb"):   # empty programs of a mechanism without an LU
         if P not in have:
-            for suf, ctype in (("lev", "int"), ("rs", "int"), ("head", "unsigned"), ("term", "unsigned")):
-                out.append(arr(ctype, "wv_%s_%s" % (P, suf), [0]))
+            for suf in ("term", "flag", "head"):
+                out.append(arr("unsigned", "wv_%s_%s" % (P, suf), [0, 0, 0, 0],
+                               align=16 if suf == "term" else 0))
     # persistent lane solver: operand-cache schedules of the LU and
     # substitution sections (lane_cache.py), 16 pad words at the end so a
     # record's words can be loaded whole

@@ -2,37 +2,41 @@
 Rodas3 solver (csrc/wave_solver.hip), built from the tables of
 emit_tables.build_tables.
 
-In that solver one wavefront (64 lanes) integrates one cell whose state
-lives in LDS.  Every sum of the solve -- a Vdot row, a Jacobian entry, an LU
-position (row-Crout form), a row of the forward or backward substitution --
-is computed by one lane, term by term in the table order, so each value is
-rounded exactly as in the lane-per-cell solvers.  A sum whose terms become
-available at different times (an LU position needs L(k,j) and U(j,c) of
-earlier positions; a substitution row needs x of earlier rows) is split into
-chunks at dependency levels: the chunk of level l holds the next terms whose
-operands are final before level l; the partial sum is kept in the sum's own
-LDS slot between chunks.  Chunks of one level are independent; they are
-sorted by length and dealt to the 64 lanes in rounds.
+In that solver one wavefront (64 lanes) integrates one cell whose LU,
+solve vector and state live in LDS.  Every sum of the solve -- a Vdot row,
+a Jacobian entry, an LU position (row-Crout form), a row of the forward or
+backward substitution -- is computed by one lane, term by term in the table
+order, so each value is rounded exactly as in the lane-per-cell solvers.  A
+sum whose terms become available at different times (an LU position needs
+L(k,j) and U(j,c) of earlier positions; a substitution row needs x of
+earlier rows) is split into chunks at dependency levels: the chunk of level
+l holds the next terms whose operands are final before level l; the partial
+sum is kept in the sum's own LDS slot between chunks.  Chunks of one level
+are independent; they are sorted by length and dealt to the 64 lanes in
+rounds.  A round's steps are padded to whole blocks of BSTEP steps.
 
-Encoding (all LDS addresses are element indices of the solver's LDS block,
-whose layout `layout()` fixes):
-  <P>_lev[NLEV+1]   first round of each level (levels are separated by a
-                    wave-wide LDS ordering point)
-  <P>_rs[NR+1]      first step of each round (steps of a round = its longest
-                    chunk; shorter chunks are padded with exact no-op terms)
-  <P>_head[(NR+1)*64] per round and lane: low 16 bits the destination slot
-                    (0xFFFF: lane idle), high 16 bits the divisor slot + 1
-                    (LU and backward substitution: the final chunk divides;
-                    Jacobian: 1 if the slot is a diagonal entry); one idle
-                    round ends the array (the solver loads a round ahead)
-  <P>_term[(NS+8)*64] per step and lane: low 16 bits operand a, high 16 bits
-                    operand b; the step is acc = fma(b, a, acc) for Vdot and
-                    Jacobian (b a coefficient slot) and acc = fma(-a, b, acc)
-                    for LU and substitution.  A pad term has a = ZERO and b
-                    = the -0.0 coefficient (Vdot, Jacobian) or ZERO: its
-                    product is -0.0, which leaves every acc unchanged.
-                    Eight steps of pads end the array (the solver loads
-                    eight steps at a time).
+Operand spaces.  fv (Vdot) and jv (Jacobian): operand a is an element of
+the wave's reaction scratch in global memory (A(r) or the partials;
+element NAB is a +0.0 pad slot), operand b an element of the mechanism's
+coefficient table (element 0 is -0.0); a step is acc = fma(b, a, acc) from
+acc = +0.0.  lu, sf, sb: both operands are LDS elements; a step is
+acc = fma(-a, b, acc) from acc = the destination slot.
+
+Encoding (the solver reads BYTE offsets, element index * 8; the LDS block
+is below 64 KB, so every offset fits 16 bits):
+  wv_<P>_term  [NB + pad][64][BSTEP] words, block-major then lane: a | b << 16;
+               a pad step reads (ZERO, ZERO) in LDS phases and (NAB, 0) in
+               fv/jv -- products -0.0 that leave every acc unchanged.  The
+               array ends with PF whole pad blocks (the solver loads PF
+               blocks ahead) and NB is padded to a multiple of PF.
+  wv_<P>_flag  [(NB + pad)/8] words: 4 bits per block (block k in bits
+               4(k%8)..4(k%8)+3): 1 first block of a round, 2 last block of a
+               round, 4 the round ends a level (an ordering point follows).
+  wv_<P>_head  [(NR + HPAD)][64] words per round and lane: destination byte
+               offset | divisor byte offset << 16 (0xFFFF: none; jv: 1 for
+               a diagonal entry); destination 0xFFFF: lane idle; HPAD idle
+               rounds end the array (the solver loads heads ahead).
+  WV_<P>_NB    blocks (a multiple of PF), WV_<P>_NR rounds.
 Phases P: fv (Vdot), jv (Jacobian in LU layout), lu (row-Crout), sf
 (forward substitution), sb (backward substitution).
 """
@@ -42,19 +46,32 @@ LANES = 64
 IDLE = 0xFFFF
 
 
-def layout(m, t, ncoef):
-    """LDS element offsets of the cell-per-wave solver for one mechanism."""
+BSTEP = 4    # steps per block (one 16-byte load of term words per lane)
+PF = 8       # blocks the solver loads ahead; NB is a multiple of it
+HPAD = 4     # idle rounds after the last (the solver loads heads ahead)
+
+
+def layout(m, t):
+    """Element offsets of the cell-per-wave solver's LDS block (LU, a +0.0
+    slot, the solve vector X -- also the stage point Ynew and the scaled
+    error terms --, the state Y) and of a wave's global scratch (rate
+    constants, reaction products / partials, a +0.0 slot, Fcn0, the four
+    stage vectors K)."""
     nvar, nspec, nreact = m["nvar"], m["nspec"], m["nreact"]
     lu_nz = m["lu_nonzero"] if m["lu_irow"] else 1
-    scr = max(nreact, len(t["pt_r"]))
+    nab = max(nreact, len(t["pt_r"]))
     L = {"LU0": 0, "ZERO": lu_nz}
     L["X0"] = lu_nz + 1
     L["Y0"] = L["X0"] + nvar
-    L["YN0"] = L["Y0"] + nspec
-    L["AB0"] = L["YN0"] + nvar
-    L["COEF0"] = L["AB0"] + scr
-    L["SH_SIZE"] = L["COEF0"] + ncoef
-    assert L["SH_SIZE"] < IDLE
+    L["SH_SIZE"] = L["Y0"] + nspec
+    assert 8 * L["SH_SIZE"] < IDLE
+    L["G_RCT"] = 0
+    L["G_AB"] = nreact
+    L["NAB"] = nab
+    L["G_F0"] = nreact + nab + 1
+    L["G_K"] = L["G_F0"] + nvar
+    L["G_SIZE"] = L["G_K"] + 4 * nvar
+    assert 8 * (nab + 1) < IDLE
     return L
 
 
@@ -78,24 +95,6 @@ def _rounds(levels):
             rs.append(rs[-1] + ns)
         lev.append(len(rs) - 1)
     return lev, rs, head, term
-
-
-BLOCK = 8   # steps the solver loads at once; the term array ends with one block of pads
-
-
-def _encode_terms(term_rows, pad):
-    out = []
-    for row in term_rows:
-        for ab in row:
-            a, b = ab if ab is not None else pad
-            out.append(a | (b << 16))
-    return out
-
-
-def _encode_terms_padded(term_rows, pad):
-    """Term words plus BLOCK steps of pads, so the solver may load a whole
-    block past any round's last step."""
-    return _encode_terms(term_rows, pad) + [pad[0] | (pad[1] << 16)] * (BLOCK * LANES)
 
 
 def _split(terms_ready, final_extra_level):
@@ -123,7 +122,8 @@ def _split(terms_ready, final_extra_level):
 
 
 def build_wave_program(m, t):
-    """Wave-program tables and LDS layout for one mechanism."""
+    """Wave-program tables and LDS layout for one mechanism.  Operands are
+    element indices of their space (see the module docstring)."""
     nvar = m["nvar"]
     crow, diag, icol = m["lu_crow"], m["lu_diag"], m["lu_icol"]
     has_lu = bool(m["lu_irow"])
@@ -134,29 +134,31 @@ def build_wave_program(m, t):
         if key not in cidx:
             cidx[key] = len(coefs)
             coefs.append(key)
-    L = layout(m, t, len(coefs))
-    ZERO, X0, AB0, C0 = L["ZERO"], L["X0"], L["AB0"], L["COEF0"]
+    L = layout(m, t)
+    ZERO, X0, NAB = L["ZERO"], L["X0"], L["NAB"]
     out = {"layout": L, "coef": coefs}
 
-    # Vdot: one chunk per row, all in level 0
+    # Vdot: one chunk per row, all in level 0; operands (reaction product,
+    # coefficient)
     fv_ptr, fv_r, fv_c = t["fv_ptr"], t["fv_r"], t["fv_c"]
     ch = []
     for i in range(nvar):
-        ch.append((X0 + i, 0, [(AB0 + fv_r[k], C0 + cidx[float(fv_c[k])])
+        ch.append((X0 + i, 0, [(fv_r[k], cidx[float(fv_c[k])])
                                for k in range(fv_ptr[i], fv_ptr[i + 1])]))
-    out["fv"] = _rounds([ch]), (ZERO, C0)
+    out["fv"] = _rounds([ch]), (NAB, 0)
 
     if not has_lu:
         return out
     nz = m["lu_nonzero"]
-    # Jacobian entries in LU layout (sign and diagonal applied at the store)
+    # Jacobian entries in LU layout (sign and diagonal applied at the store);
+    # operands (partial, coefficient)
     jv_ptr, jv_b, jv_c = t["jv_ptr"], t["jv_b"], t["jv_c"]
     isdiag = set(diag[:nvar])
     ch = []
     for q in range(nz):
-        ch.append((q, 1 if q in isdiag else 0, [(AB0 + jv_b[k], C0 + cidx[float(jv_c[k])])
+        ch.append((q, 1 if q in isdiag else 0, [(jv_b[k], cidx[float(jv_c[k])])
                                                 for k in range(jv_ptr[q], jv_ptr[q + 1])]))
-    out["jv"] = _rounds([ch]), (ZERO, C0)
+    out["jv"] = _rounds([ch]), (NAB, 0)
 
     # row-Crout LU: position q = (k, c) sums -L(k,j)*U(j,c) in cr order, then
     # divides by U(c,c) when c < k
@@ -198,47 +200,113 @@ def build_wave_program(m, t):
     return out
 
 
+def encode(prog, P):
+    """The solver's arrays of phase P (see the module docstring): term words
+    [NB + PF][64][BSTEP], flag words, head words, NB, NR."""
+    (lev, rs, head, term_rows), pad = prog[P]
+    nr = len(rs) - 1
+    lds = P not in ("fv", "jv")
+    blocks, flags = [], []
+    lev_end = set(lev[1:])
+    for r in range(nr):
+        rows = term_rows[rs[r]:rs[r + 1]]
+        nblk = max(1, -(-len(rows) // BSTEP))
+        rows = rows + [[None] * LANES] * (nblk * BSTEP - len(rows))
+        for b in range(nblk):
+            blocks.append(rows[b * BSTEP:(b + 1) * BSTEP])
+            f = (1 if b == 0 else 0) | (2 if b == nblk - 1 else 0)
+            if b == nblk - 1 and (r + 1) in lev_end:
+                f |= 4
+            flags.append(f)
+    nb = len(blocks)
+    nb_pad = -(-nb // PF) * PF
+    empty = [[None] * LANES] * BSTEP
+    blocks += [empty] * (nb_pad - nb + PF)
+    flags += [0] * (nb_pad - nb + PF)
+
+    def word(ab):
+        a, b = ab if ab is not None else pad
+        return (8 * int(a)) | ((8 * int(b)) << 16)
+    term = []
+    for blk in blocks:
+        for ln in range(LANES):
+            for st in range(BSTEP):
+                term.append(word(blk[st][ln]))
+    flag = []
+    for k in range(0, len(flags), 8):
+        w = 0
+        for i, f in enumerate(flags[k:k + 8]):
+            w |= f << (4 * i)
+        flag.append(w)
+
+    def hword(h):
+        h = int(h)
+        d, hi = h & 0xFFFF, h >> 16
+        if d == IDLE:
+            return IDLE
+        if P in ("lu", "sb"):
+            hi = 8 * (hi - 1) if hi else IDLE
+        return (8 * d) | (hi << 16)
+    heads = [hword(h) for h in head] + [IDLE] * (HPAD * LANES)
+    return term, flag, heads, nb_pad, nr
+
+
 def emit_arrays(prog):
-    """(name, ctype, values) of the wave-program tables."""
-    arrays = []
+    """(name, ctype, values) of the wave-program tables, and the constants
+    (name, value)."""
+    arrays, consts = [], []
     for P in ("fv", "jv", "lu", "sf", "sb"):
         if P not in prog:
             continue
-        (lev, rs, head, term_rows), pad = prog[P]
-        arrays.append(("wv_%s_lev" % P, "int", lev))
-        arrays.append(("wv_%s_rs" % P, "int", rs))
-        arrays.append(("wv_%s_head" % P, "unsigned", head + [IDLE] * LANES))
-        arrays.append(("wv_%s_term" % P, "unsigned", _encode_terms_padded(term_rows, pad)))
+        term, flag, heads, nb, nr = encode(prog, P)
+        arrays.append(("wv_%s_term" % P, "unsigned", term))
+        arrays.append(("wv_%s_flag" % P, "unsigned", flag))
+        arrays.append(("wv_%s_head" % P, "unsigned", heads))
+        consts.append(("WV_%s_NB" % P.upper(), nb))
+        consts.append(("WV_%s_NR" % P.upper(), nr))
     arrays.append(("wv_coef", "double", prog["coef"]))
-    return arrays
+    return arrays, consts
 
 
 # ---------------------------------------------------------------------------
 # Emulation (tests): run a phase's program on a flat LDS image with a given
 # arithmetic for one step, to show it performs the same operations in the
 # same order as the sequential table walk.
-def run_phase(prog, P, sh, step, init_from_dest, store):
-    """Executes phase P on the array sh (modified in place).  step(acc, a, b)
-    -> acc; init_from_dest: acc starts at sh[dest] (else +0.0); store(sh,
-    dest, hi, acc, final_hi) writes the result."""
-    (lev, rs, head, term_rows), pad = prog[P]
-    for l in range(len(lev) - 1):
-        writes = []
-        for r in range(lev[l], lev[l + 1]):
-            for ln in range(LANES):
-                h = head[r * LANES + ln]
-                dest = h & 0xFFFF
-                if dest == IDLE:
-                    continue
-                acc = sh[dest] if init_from_dest else 0.0
-                for s in range(rs[r], rs[r + 1]):
-                    ab = term_rows[s][ln]
-                    a, b = ab if ab is not None else pad
-                    acc = step(acc, sh[a], sh[b])
-                writes.append((dest, h >> 16, acc))
-        # a level's results become visible only after the level
-        for dest, hi, acc in writes:
-            store(sh, dest, hi, acc)
+def run_phase(prog, P, sh, step, init_from_dest, store, ab=None, coef=None):
+    """Executes phase P from its encoded arrays as the solver does: block by
+    block, every lane's four steps, a round's result stored at its last
+    block.  sh: the LDS block (element list, modified in place); ab, coef:
+    the reaction scratch (with its +0.0 pad slot) and the coefficient table
+    (fv, jv operands).  step(acc, a, b) -> acc; init_from_dest: acc starts
+    at sh[dest] (else +0.0); store(sh, dest, hi, acc) writes the result
+    (element indices; hi: divisor element + 1, or the Jacobian's diagonal
+    flag, 0 for none)."""
+    term, flag, heads, nb, nr = encode(prog, P)
+    lds = P not in ("fv", "jv")
+    r = 0
+    acc = [0.0] * LANES
+    for k in range(nb):
+        f = (flag[k // 8] >> (4 * (k % 8))) & 15
+        for ln in range(LANES):
+            h = heads[r * LANES + ln]
+            dest = h & 0xFFFF
+            if f & 1:
+                acc[ln] = sh[dest // 8] if (init_from_dest and dest != IDLE) else 0.0
+            for st in range(BSTEP):
+                w = term[(k * LANES + ln) * BSTEP + st]
+                a, b = w & 0xFFFF, w >> 16
+                if lds:
+                    acc[ln] = step(acc[ln], sh[a // 8], sh[b // 8])
+                else:
+                    acc[ln] = step(acc[ln], ab[a // 8], coef[b // 8])
+            if f & 2 and dest != IDLE:
+                hi = h >> 16
+                if P in ("lu", "sb"):
+                    hi = 0 if hi == IDLE else hi // 8 + 1
+                store(sh, dest // 8, hi, acc[ln])
+        if f & 2:
+            r += 1
+    assert r == nr
     return sh
 
 
@@ -270,7 +338,7 @@ def level_dependencies_ok(prog, P):
                     reads.append((h >> 16) - 1)
                 for s in range(rs[r], rs[r + 1]):
                     ab = term_rows[s][ln]
-                    if ab is not None:
+                    if ab is not None and P not in ("fv", "jv"):   # fv, jv read global memory
                         reads += [ab[0], ab[1]]
                 for o in reads:
                     if o in last and last[o] >= l:

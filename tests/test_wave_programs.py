@@ -29,12 +29,14 @@ def programs(m):
 
 
 def fresh_lds(prog, rng):
+    """Random LDS block (the +0.0 slot set) and reaction scratch (its +0.0
+    pad slot set) and the coefficient table."""
     L = prog["layout"]
     sh = list(rng.uniform(-1.0, 1.0, L["SH_SIZE"]))
     sh[L["ZERO"]] = 0.0
-    for k, c in enumerate(prog["coef"]):
-        sh[L["COEF0"] + k] = c
-    return sh, L
+    ab = list(rng.uniform(-1.0, 1.0, L["NAB"] + 1))
+    ab[L["NAB"]] = 0.0
+    return sh, L, ab, list(prog["coef"])
 
 
 @pytest.mark.parametrize("name", ["fullchem", "Hg"])
@@ -66,29 +68,29 @@ def test_programs_reproduce_table_walk(name, mech_of):
     crow, diag, icol = m.lu_crow, m.lu_diag, m.lu_icol
 
     # Vdot: A in the scratch, rows into X
-    sh, L = fresh_lds(prog, rng)
-    AB0, X0 = L["AB0"], L["X0"]
+    sh, L, ab, coef = fresh_lds(prog, rng)
+    X0 = L["X0"]
     ref = []
     for i in range(nv):
         acc = 0.0
         for k in range(t["fv_ptr"][i], t["fv_ptr"][i + 1]):
-            acc = acc + float(t["fv_c"][k]) * sh[AB0 + t["fv_r"][k]]
+            acc = acc + float(t["fv_c"][k]) * ab[t["fv_r"][k]]
         ref.append(acc)
 
     def st_plain(s, d, hi, acc):
         s[d] = acc
-    W.run_phase(prog, "fv", sh, lambda acc, a, b: acc + b * a, False, st_plain)
+    W.run_phase(prog, "fv", sh, lambda acc, a, b: acc + b * a, False, st_plain, ab=ab, coef=coef)
     assert sh[X0:X0 + nv] == ref
 
     # Jacobian: partials in the scratch, Ghimj = -JVS + diag into the LU block
-    sh, L = fresh_lds(prog, rng)
+    sh, L, ab, coef = fresh_lds(prog, rng)
     dg = 37.5
     ref = []
     dset = set(diag[:nv])
     for q in range(nz):
         acc = 0.0
         for k in range(t["jv_ptr"][q], t["jv_ptr"][q + 1]):
-            acc = acc + float(t["jv_c"][k]) * sh[AB0 + t["jv_b"][k]]
+            acc = acc + float(t["jv_c"][k]) * ab[t["jv_b"][k]]
         v = -1.0 * acc
         if q in dset:
             v = v + dg
@@ -99,11 +101,11 @@ def test_programs_reproduce_table_walk(name, mech_of):
         if hi:
             v = v + dg
         s[d] = v
-    W.run_phase(prog, "jv", sh, lambda acc, a, b: acc + b * a, False, st_jac)
+    W.run_phase(prog, "jv", sh, lambda acc, a, b: acc + b * a, False, st_jac, ab=ab, coef=coef)
     assert sh[:nz] == ref
 
     # LU (row-Crout, the lane solvers' cr tables) on a diagonally dominant matrix
-    sh, L = fresh_lds(prog, rng)
+    sh, L, _, _ = fresh_lds(prog, rng)
     for k in range(nv):
         sh[diag[k]] += 40.0
     lu = list(sh[:nz])
@@ -143,29 +145,45 @@ def test_emitted_tables_match_programs(fullchem):
     hdr = (ROOT / "geos-chem_amd" / "csrc" / "gen" / "mech_fullchem.h").read_text()
     L = prog["layout"]
     assert "WV_X0 = %d," % L["X0"] in hdr and "WV_SH = %d;" % L["SH_SIZE"] in hdr
-    for name, ctype, vals in W.emit_arrays(prog):
+    arrays, consts = W.emit_arrays(prog)
+    for name, ctype, vals in arrays:
         assert "%s %s[%d]" % (ctype, name, len(vals)) in hdr, name
+    for name, v in consts:
+        assert "%s = %d" % (name, v) in hdr, name
 
 
 @pytest.mark.parametrize("name", ["fullchem", "Hg", "carbon"])
-def test_program_indices_inside_lds_block(name, mech_of):
-    """Every slot a program names (destination, divisor, both operands of
-    every step, pads included) lies inside the mechanism's LDS block, and
-    destinations never touch the constant slots (ZERO, coefficients)."""
+def test_program_offsets_inside_their_blocks(name, mech_of):
+    """Every byte offset an encoded program names -- destination, divisor,
+    both operands of every step, pads included -- lies inside its block: the
+    LDS block for LU and substitution operands and every destination, the
+    reaction scratch (with its pad slot) and the coefficient table for the
+    Vdot and Jacobian operands; destinations never touch the +0.0 slot or
+    the state Y; the block flags mark every round's first and last block;
+    the term array holds PF pad blocks past NB, NB a multiple of PF."""
     W, t, prog = programs(mech_of(name))
     L = prog["layout"]
+    sh8, ab8, cf8 = 8 * L["SH_SIZE"], 8 * (L["NAB"] + 1), 8 * len(prog["coef"])
     for P in ("fv", "jv", "lu", "sf", "sb"):
         if P not in prog:
             continue
-        (lev, rs, head, term_rows), pad = prog[P]
-        assert len(head) == 64 * (len(rs) - 1) and len(term_rows) == rs[-1]
-        assert lev[0] == 0 and lev[-1] == len(rs) - 1 and list(lev) == sorted(lev)
-        for h in head:
+        term, flag, heads, nb, nr = W.encode(prog, P)
+        assert nb % W.PF == 0 and len(term) == (nb + W.PF) * 64 * W.BSTEP
+        assert len(heads) == (nr + W.HPAD) * 64 and len(flag) * 8 >= nb + W.PF
+        fl = [(flag[k // 8] >> (4 * (k % 8))) & 15 for k in range(nb + W.PF)]
+        assert sum(f & 1 for f in fl) == nr and sum((f >> 1) & 1 for f in fl) == nr
+        assert all(f == 0 for f in fl[nb:])
+        for h in heads:
             d, hi = h & 0xFFFF, h >> 16
             if d == W.IDLE:
                 continue
-            assert d < L["ZERO"] or L["X0"] <= d < L["COEF0"]
-            if P in ("lu", "sb") and hi:
-                assert hi - 1 < L["ZERO"]
-        for row in W._encode_terms(term_rows, pad):
-            assert (row & 0xFFFF) < L["SH_SIZE"] and (row >> 16) < L["SH_SIZE"]
+            assert d % 8 == 0 and (d < 8 * L["ZERO"] or 8 * L["X0"] <= d < 8 * L["Y0"])
+            if P in ("lu", "sb") and hi != W.IDLE:
+                assert hi % 8 == 0 and hi < 8 * L["ZERO"]
+        for w in term:
+            a, b = w & 0xFFFF, w >> 16
+            assert a % 8 == 0 and b % 8 == 0
+            if P in ("fv", "jv"):
+                assert a < ab8 and b < cf8
+            else:
+                assert a < sh8 and b < sh8
