@@ -68,6 +68,9 @@ def parse():
                    help="skip the end-to-end leg (host arrays through gckpp_fullchem_step)")
     p.add_argument("--max-waves", type=int, default=0,
                    help="cap on the waves of a persistent launch (0: every resident wave)")
+    p.add_argument("--cpu-box", type=int, default=0, metavar="THREADS",
+                   help="only time the compiled CPU reference on the 4,096 synthetic box cells "
+                        "(seed 0) with this many OpenMP threads, median of 5 (no GPU needed)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device copy (reference for HBM byte counters)")
     return p.parse_args()
@@ -117,6 +120,34 @@ def cpu_baseline(mech, w, ic, rc, nsample, reps=5):
                       "chem step; median of %d repetitions (%.2f s each) after a warm-up"
                       % (nsample, w.name, max(1, w.ncell // nsample), reps, dt)}
     return base, idx, (c_ref, ist_ref, e_ref)
+
+
+def cpu_box(threads, reps=5):
+    """The compiled reference's Do_FullChem step on the 4,096 box cells
+    (seed 0, cold start) with a stated thread count: the reference's own
+    CPU case, timed as the reference runs it (OpenMP DYNAMIC,24)."""
+    import numpy as np
+    from gckpp_amd import mechanism, options, synth
+    from oracle import ref
+    m = mechanism.load("fullchem")
+    w = synth.box_cells(m, 4096, 0)
+    ic, rc = options.fullchem_icntrl()
+    R = ref.RefKPP(m)
+    R.fullchem_step(w.subset(np.arange(64)), ic, rc, nthreads=threads)   # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, ist, _, e = R.fullchem_step(w, ic, rc, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+        assert (e == 1).all()
+    dt = float(np.median(times))
+    print(json.dumps({"metric": "fullchem cells/sec per 1200 s chem step, CPU reference",
+                      "value": round(w.ncell / dt, 2), "unit": "cells/s", "threads": threads,
+                      "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+                      "seconds_per_rep": [round(t, 3) for t in times],
+                      "mean_attempts_per_cell": round(float(ist[2].mean()), 2),
+                      "config": {"workload": "4,096 synthetic box cells (seed 0), cold start, "
+                                             "dt 1200 s, repo RTOL/ATOL"}}), flush=True)
 
 
 def cpu_model():
@@ -823,6 +854,8 @@ def main():
     args = parse()
     if args.copy_calibration:
         return copy_calibration()
+    if args.cpu_box:
+        return cpu_box(args.cpu_box)
     if args.gpus != 1:
         print("bench.py measures one GPU: the chemistry step has no cross-GPU communication "
               "(cells shard independently), so run it with --gpus 1", file=sys.stderr)

@@ -93,6 +93,15 @@ struct gckpp_ctx {
 
 extern "C" const char *gckpp_last_error(void) { return g_err; }
 
+namespace gckpp {
+// The message of a failure detected outside this file (host-side file I/O);
+// an empty message clears it.  Returns -1.
+int set_error(const char *msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return -1;
+}
+}  // namespace gckpp
+
 static bool mech_ok(int mech) {
   return mech == MECH_FULLCHEM || mech == MECH_HG || mech == MECH_CARBON;
 }
@@ -483,7 +492,7 @@ static int comp(gckpp_ctx *c, int which, int64_t ncell, const double *in, const 
   if (!c) return fail("null context");
   if (ncell <= 0) return 0;
   if (ncell > c->wstride) return fail("ncell=%lld exceeds max_cells", (long long)ncell);
-  if (c->mech == MECH_CARBON && which != 0)
+  if (c->mech == MECH_CARBON && which != 0 && which != 4)
     return fail("the carbon mechanism has no sparse Jacobian (forward Euler)");
   CompArgs a{};
   a.ncell = ncell;
@@ -516,6 +525,10 @@ extern "C" int gckpp_decomp_dev(gckpp_ctx *c, int64_t ncell, double *jvs, int32_
 extern "C" int gckpp_solve_dev(gckpp_ctx *c, int64_t ncell, const double *lu, double *x,
                                void *stream) {
   return comp(c, 3, ncell, lu, nullptr, x, nullptr, stream);
+}
+extern "C" int gckpp_aout_dev(gckpp_ctx *c, int64_t ncell, const double *conc, const double *rct,
+                              double *aout, void *stream) {
+  return comp(c, 4, ncell, conc, rct, aout, nullptr, stream);
 }
 
 extern "C" int gckpp_rates_dev(gckpp_ctx *c, int64_t ncell, const double *met,
@@ -776,6 +789,27 @@ extern "C" int gckpp_rates(gckpp_ctx *c, int64_t ncell, const double *met, const
     if (rc) return rc;
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(stage_out(rconst, v.rconst, c->info.nreact, n, ncell, c0));
+  }
+  return 0;
+}
+
+// Host-pointer Aout (Fun's reaction rates): per chunk, conc and rconst in,
+// the rates computed in place of the staged rconst, copied out.
+extern "C" int gckpp_aout(gckpp_ctx *c, int64_t ncell, const double *conc, const double *rconst,
+                          double *aout) {
+  if (!c) return fail("gckpp_aout: null context");
+  if (ncell <= 0) return ncell == 0 ? 0 : fail("gckpp_aout: negative ncell");
+  if (!conc || !rconst || !aout) return fail("gckpp_aout: null argument");
+  if (host_stage(c, "gckpp_aout")) return -1;
+  for (int64_t c0 = 0; c0 < ncell; c0 += c->wstride) {
+    const int64_t n = ncell - c0 < c->wstride ? ncell - c0 : c->wstride;
+    const Stage v = stage_views(c, n);
+    HIP_OK(stage_in(v.conc, conc, c->info.nspec, n, ncell, c0));
+    HIP_OK(stage_in(v.rconst, rconst, c->info.nreact, n, ncell, c0));
+    int rc = comp(c, 4, n, v.conc, v.rconst, v.rconst, nullptr, nullptr);
+    if (rc) return rc;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(stage_out(aout, v.rconst, c->info.nreact, n, ncell, c0));
   }
   return 0;
 }

@@ -109,6 +109,7 @@ struct CompArgs {
 };
 
 MechInfo mech_info(int mech);
+int set_error(const char *msg);  // gckpp_last_error's message; returns -1
 // doubles of per-cell workspace the integrator needs
 inline int64_t integrate_work_per_cell(const MechInfo &m) {
   int64_t scr = m.nreact > m.npart ? m.nreact : m.npart;

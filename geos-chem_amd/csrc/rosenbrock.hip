@@ -1348,6 +1348,28 @@ __global__ __launch_bounds__(64) void k_solve(CompArgs a) {
   dev_solve<M>(CCol{a.in + cell, a.istride}, Col{a.out + cell, a.ostride});
 }
 
+// Fun's reaction rates A(r) = RCT(r) * product of the reactants (the Aout
+// argument of KPP's Fun, KPP/fullchem/gckpp_Function.F90:51-2152), in the
+// reactant order of the Vdot evaluation.
+template <class M>
+__global__ __launch_bounds__(64) void k_aout(CompArgs a) {
+  const int64_t cell = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (cell >= a.ncell) return;
+  CCol y{a.in + cell, a.istride};
+  CCol r{a.rconst + cell, a.rstride};
+#pragma unroll 2
+  for (int k = 0; k < M::NREACT; ++k) {
+    double v = r[k];
+#pragma unroll
+    for (int q = 0; q < M::RX_MAX; ++q) {
+      const int s = M::rx_s(k * M::RX_MAX + q);
+      if (s < 0) break;
+      v *= spc<M>(y, y, s);
+    }
+    a.out[(int64_t)k * a.ostride + cell] = v;
+  }
+}
+
 // Host-side launchers ---------------------------------------------------------
 template <class M>
 static void launch_integrate(const IntegrateArgs &a, hipStream_t s) {
@@ -1362,6 +1384,7 @@ static void launch_comp(int which, const CompArgs &a, hipStream_t s) {
     case 1: hipLaunchKernelGGL(k_jac<M>, grid, dim3(64), 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_decomp<M>, grid, dim3(64), 0, s, a); break;
     case 3: hipLaunchKernelGGL(k_solve<M>, grid, dim3(64), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_aout<M>, grid, dim3(64), 0, s, a); break;
   }
 }
 
@@ -1467,6 +1490,8 @@ void launch_comp_mech(int mech, int which, const CompArgs &a, hipStream_t s) {
   else if (mech == MECH_HG) launch_comp<Traits_Hg>(which, a, s);
   else if (mech == MECH_CARBON && which == 0)
     hipLaunchKernelGGL(k_fun<Traits_carbon>, dim3((unsigned)((a.ncell + 63) / 64)), dim3(64), 0, s, a);
+  else if (mech == MECH_CARBON && which == 4)
+    hipLaunchKernelGGL(k_aout<Traits_carbon>, dim3((unsigned)((a.ncell + 63) / 64)), dim3(64), 0, s, a);
 }
 
 MechInfo mech_info(int mech) {

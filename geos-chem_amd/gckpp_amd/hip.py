@@ -37,6 +37,8 @@ SYMBOLS = [
     "gckpp_batch_set_handoff",
     "gckpp_batch_handoff_count",
     "gckpp_persistent_handoff_stats",
+    "gckpp_aout_dev", "gckpp_aout", "gckpp_kppsa_write", "gckpp_kppsa_read",
+    "gckpp_kpph_restart_write", "gckpp_kpph_restart_read",
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
@@ -53,6 +55,20 @@ class Part(ctypes.Structure):
                 ("icntrl", ctypes.c_int32 * 20), ("rcntrl", ctypes.c_double * 20),
                 ("order", ctypes.c_void_p)]
 SOLVER_PLAIN, SOLVER_PERSISTENT = 0, 1
+NAME_LEN = 64    # GCKPP_NAME_LEN
+
+
+class KppsaMeta(ctypes.Structure):
+    """gckpp_kppsa_meta: the metadata and integrator fields of a
+    KPP-Standalone state file."""
+    _fields_ = [("location", ctypes.c_char * 256), ("timestamp", ctypes.c_char * 64),
+                ("lon", ctypes.c_double), ("lat", ctypes.c_double), ("level", ctypes.c_int32),
+                ("pressure_hpa", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("airnumden", ctypes.c_double), ("h2o_vmr", ctypes.c_double),
+                ("cloud_frac", ctypes.c_double), ("suncos", ctypes.c_double),
+                ("h_init", ctypes.c_double), ("h_exit", ctypes.c_double), ("dt", ctypes.c_double),
+                ("nsteps", ctypes.c_int32), ("icntrl", ctypes.c_int32 * 20),
+                ("rcntrl", ctypes.c_double * 20)]
 
 
 class GckppError(RuntimeError):
@@ -92,6 +108,15 @@ def load():
         getattr(L, f).argtypes = [vp, i64, dp, dp, dp, vp]
     L.gckpp_decomp_dev.argtypes = [vp, i64, dp, i32p, vp]
     L.gckpp_solve_dev.argtypes = [vp, i64, dp, dp, vp]
+    L.gckpp_aout_dev.argtypes = [vp, i64, dp, dp, dp, vp]
+    L.gckpp_aout.argtypes = [vp, i64, dp, dp, dp]
+    L.gckpp_kppsa_write.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(KppsaMeta),
+                                    ctypes.c_char_p, dp, dp, dp, dp]
+    L.gckpp_kpph_restart_write.argtypes = [ctypes.c_char_p, i64, i64, i64, dp, ctypes.c_char_p]
+    L.gckpp_kpph_restart_read.argtypes = [ctypes.c_char_p, i64, i64, i64, dp,
+                                          ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p]
+    L.gckpp_kppsa_read.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(KppsaMeta),
+                                   ctypes.c_char_p, dp, dp, dp, dp]
     L.gckpp_step_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.gckpp_step_failures.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.gckpp_set_solver.argtypes = [vp, ctypes.c_int]
@@ -483,6 +508,15 @@ class Context:
                "gckpp_solve_dev")
         return out
 
+    def aout(self, conc, rconst):
+        """Fun's reaction rates A[NREACT][ncell]."""
+        self._comp_in(conc, rconst)
+        out = self.torch.empty((self.mech.nreact, conc.shape[1]), dtype=self.torch.float64,
+                               device=conc.device)
+        _check(self.L.gckpp_aout_dev(self.h, conc.shape[1], _ptr(conc), _ptr(rconst), _ptr(out),
+                                     _stream()), "gckpp_aout_dev")
+        return out
+
 
 class Batch:
     """gckpp_batch: one persistent launch per chemistry step over the cells of
@@ -556,6 +590,68 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def kppsa_write(mech, path, meta, conc, atol, rconst, aout, names=None):
+    """Writes one cell's KPP-Standalone state file (host numpy arrays:
+    conc[NSPEC], atol[NSPEC], rconst[NREACT], aout[NREACT]; names: NSPEC
+    strings, None for "C<n>" lines)."""
+    import numpy as np
+    L = load()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (conc, atol, rconst, aout)]
+    for a, n in zip(arrs, (mech.nspec, mech.nspec, mech.nreact, mech.nreact)):
+        if a.shape != (n,):
+            raise GckppError("kppsa_write: array of shape %s, expected (%d,)" % (a.shape, n))
+    nb = None
+    if names is not None:
+        if len(names) != mech.nspec:
+            raise GckppError("kppsa_write: %d names for %d species" % (len(names), mech.nspec))
+        nb = b"".join((n or "").encode()[:NAME_LEN - 1].ljust(NAME_LEN, b"\0") for n in names)
+    _check(L.gckpp_kppsa_write(mech.mech_id, str(path).encode(), ctypes.byref(meta), nb,
+                               *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs]), "gckpp_kppsa_write")
+
+
+def kppsa_read(mech, path):
+    """Reads a KPP-Standalone state file: (meta, names, conc, atol, rconst, aout)."""
+    import numpy as np
+    L = load()
+    meta = KppsaMeta()
+    nb = ctypes.create_string_buffer(mech.nspec * NAME_LEN)
+    conc, atol = np.zeros(mech.nspec), np.zeros(mech.nspec)
+    rct, aout = np.zeros(mech.nreact), np.zeros(mech.nreact)
+    _check(L.gckpp_kppsa_read(mech.mech_id, str(path).encode(), ctypes.byref(meta), nb,
+                              *[a.ctypes.data_as(ctypes.c_void_p) for a in (conc, atol, rct, aout)]),
+           "gckpp_kppsa_read")
+    raw = nb.raw
+    names = [raw[i * NAME_LEN:(i + 1) * NAME_LEN].split(b"\0", 1)[0].decode() for i in range(mech.nspec)]
+    return meta, names, conc, atol, rct, aout
+
+
+def kpph_restart_write(path, kpph, model_tag=""):
+    """Writes KPPHvalue (numpy array (nx, ny, nz), any memory order) as the
+    restart variable Chem_KPPHvalue."""
+    import numpy as np
+    L = load()
+    k = np.asfortranarray(kpph, dtype=np.float64)
+    nx, ny, nz = k.shape
+    flat = np.ascontiguousarray(k.reshape(-1, order="F"))
+    _check(L.gckpp_kpph_restart_write(str(path).encode(), nx, ny, nz,
+                                      flat.ctypes.data_as(ctypes.c_void_p), model_tag.encode()),
+           "gckpp_kpph_restart_write")
+
+
+def kpph_restart_read(path, shape):
+    """(KPPHvalue (nx, ny, nz), found, model tag) from a restart file."""
+    import numpy as np
+    L = load()
+    nx, ny, nz = shape
+    flat = np.zeros(nx * ny * nz)
+    found = ctypes.c_int32()
+    tag = ctypes.create_string_buffer(NAME_LEN)
+    _check(L.gckpp_kpph_restart_read(str(path).encode(), nx, ny, nz,
+                                     flat.ctypes.data_as(ctypes.c_void_p), ctypes.byref(found), tag),
+           "gckpp_kpph_restart_read")
+    return flat.reshape((nx, ny, nz), order="F"), bool(found.value), tag.value.decode()
 
 
 def order_by_h0(h0):

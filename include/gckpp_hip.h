@@ -144,6 +144,74 @@ int gckpp_decomp_dev(gckpp_ctx *ctx, int64_t ncell, double *jvs, int32_t *ier,
                      void *stream);
 int gckpp_solve_dev(gckpp_ctx *ctx, int64_t ncell, const double *lu, double *x,
                     void *stream);
+/* Fun's reaction rates (the Aout argument of Fun, gckpp_Function.F90:51-2152):
+ * aout[NREACT][ncell] = RCT(r) times the product of r's reactant
+ * concentrations, conc[NSPEC][ncell]; any mechanism. */
+int gckpp_aout_dev(gckpp_ctx *ctx, int64_t ncell, const double *conc,
+                   const double *rconst, double *aout, void *stream);
+/* The same with host arrays, synchronous; any ncell (chunks of at most
+ * max_cells through the staging buffers of gckpp_rates). */
+int gckpp_aout(gckpp_ctx *ctx, int64_t ncell, const double *conc, const double *rconst,
+               double *aout);
+
+/* KPP-Standalone chemical-state files, one cell per file
+ * (GeosCore/kppsa_interface_mod.F90:569-843, KppSa_Write_Samples): a header
+ * whose first line is its own line count, the cell's metadata and
+ * integrator options, then "Name,Value,ATOL" for the NSPEC species, "R<n>,"
+ * for the NREACT rate constants and "A<n>," for Fun's reaction rates, with
+ * the reference's Fortran formats (es25.16e3, es10.2e2, f11.4, ...).  Host
+ * code, no GPU.  The strings are NUL-terminated; location is the cell name
+ * of the file ("Location:"), timestamp "YYYY/MM/DD hh:mm" as written by
+ * TIMESTAMP_STRING. */
+typedef struct {
+  char location[256];
+  char timestamp[64];
+  double lon, lat;             /* degrees */
+  int32_t level;               /* GEOS-Chem vertical level (1-based) */
+  double pressure_hpa, temperature, airnumden, h2o_vmr, cloud_frac, suncos;
+  double h_init, h_exit, dt;   /* KPPHvalue before Integrate, RSTATE(Nhexit), chem step [s] */
+  int32_t nsteps;              /* ISTATUS(3) */
+  int32_t icntrl[20];
+  double rcntrl[20];
+} gckpp_kppsa_meta;
+
+/* Writes one cell's file.  names: NSPEC records of GCKPP_NAME_LEN bytes,
+ * each ended by a NUL or padded with blanks (a Fortran CHARACTER(LEN=64)
+ * array); an empty record, or a null names, writes "C<n>" as the reference
+ * does for a KPP species with no species-database entry.  conc[NSPEC] (the
+ * concentrations before Integrate), atol[NSPEC], rconst[NREACT],
+ * aout[NREACT]. */
+enum { GCKPP_NAME_LEN = 64 };
+int gckpp_kppsa_write(int mech_id, const char *path, const gckpp_kppsa_meta *meta,
+                      const char *names, const double *conc, const double *atol,
+                      const double *rconst, const double *aout);
+/* Reads one cell's file back (any of the outputs may be null; names, if
+ * given, receives the NSPEC names as NUL-terminated records of
+ * GCKPP_NAME_LEN bytes).  Fails with a message if the file's species or
+ * reaction count is not the mechanism's, or a line does not parse. */
+int gckpp_kppsa_read(int mech_id, const char *path, gckpp_kppsa_meta *meta, char *names,
+                     double *conc, double *atol, double *rconst, double *aout);
+
+/* The KPPHvalue restart variable: each cell's RSTATUS(Nhnew), the next
+ * step's Hstart, kept across runs in the restart file's Chem_KPPHvalue
+ * (registered at Headers/state_chm_mod.F90:1742-1753, read back with a
+ * zero fallback at GeosCore/hco_utilities_gc_mod.F90:2144-2177).  The
+ * writer makes a netCDF classic file (64-bit offset) with dimensions
+ * time(1), lev(nz), lat(ny), lon(nx), the double variable
+ * Chem_KPPHvalue(time,lev,lat,lon) [s] -- the Fortran array
+ * KPPHvalue(nx,ny,nz) as it lies in memory -- and the global attribute
+ * gckpp_step_model = model_tag (the step-size model the values came from,
+ * "" for none).  Host code, no GPU. */
+int gckpp_kpph_restart_write(const char *path, int64_t nx, int64_t ny, int64_t nz,
+                             const double *kpph, const char *model_tag);
+/* Reads Chem_KPPHvalue (double or float) from a netCDF classic file
+ * (CDF-1 or CDF-2) into kpph[nx*ny*nz]; the variable must have that many
+ * values.  Without the variable: *found = 0 and kpph = 0 (a cold start,
+ * as the reference does), else *found = 1.  model_tag, if not null,
+ * receives the file's gckpp_step_model attribute (GCKPP_NAME_LEN bytes,
+ * NUL-terminated; empty when absent). */
+int gckpp_kpph_restart_read(const char *path, int64_t nx, int64_t ny, int64_t nz,
+                            double *kpph, int32_t *found, char *model_tag);
 
 /* Device time [ms] of the parts of the last gckpp_fullchem_step_dev call on
  * this context, from HIP events on its stream: ms[0] Update_RCONST,

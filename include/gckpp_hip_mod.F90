@@ -12,7 +12,24 @@ MODULE gckpp_hip_mod
   PUBLIC :: gckpp_init, gckpp_finalize, gckpp_set_tolerances, gckpp_set_species_consts
   PUBLIC :: gckpp_set_solver, gckpp_fullchem_step_dev, gckpp_integrate, gckpp_last_error
   PUBLIC :: gckpp_mech_sizes, gckpp_rates, gckpp_fullchem_step, gckpp_set_persistent_handoff
-  PUBLIC :: gckpp_step_failures
+  PUBLIC :: gckpp_step_failures, gckpp_aout, gckpp_kppsa_write, gckpp_kppsa_read
+  PUBLIC :: gckpp_kpph_restart_write, gckpp_kpph_restart_read
+  INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_NAME_LEN = 64
+
+  ! gckpp_kppsa_meta: metadata and integrator fields of a KPP-Standalone
+  ! state file (GeosCore/kppsa_interface_mod.F90:569-843); strings are
+  ! NUL-terminated
+  TYPE, BIND(C), PUBLIC :: gckpp_kppsa_meta
+    CHARACTER(KIND=C_CHAR) :: location(256)
+    CHARACTER(KIND=C_CHAR) :: timestamp(64)
+    REAL(C_DOUBLE)         :: lon, lat
+    INTEGER(C_INT32_T)     :: level
+    REAL(C_DOUBLE)         :: pressure_hpa, temperature, airnumden, h2o_vmr, cloud_frac, suncos
+    REAL(C_DOUBLE)         :: h_init, h_exit, dt
+    INTEGER(C_INT32_T)     :: nsteps
+    INTEGER(C_INT32_T)     :: icntrl(20)
+    REAL(C_DOUBLE)         :: rcntrl(20)
+  END TYPE gckpp_kppsa_meta
 
   INTERFACE
     ! out = NSPEC, NVAR, NFIX, NREACT, LU_NONZERO, NMET, NHET, NPHOTOL (no GPU needed)
@@ -120,6 +137,54 @@ MODULE gckpp_hip_mod
       REAL(C_DOUBLE),     INTENT(IN)    :: rconst(*)
       INTEGER(C_INT32_T), INTENT(OUT)   :: istatus(*), ierr(*)
       REAL(C_DOUBLE),     INTENT(OUT)   :: rstatus(*)
+    END FUNCTION
+    ! Fun's reaction rates A (host arrays, synchronous): C(ncell,NSPEC),
+    ! RCONST(ncell,NREACT) -> AOUT(ncell,NREACT)
+    INTEGER(C_INT) FUNCTION gckpp_aout(ctx, ncell, conc, rconst, aout) BIND(C, name="gckpp_aout")
+      IMPORT :: C_INT, C_INT64_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR),        VALUE       :: ctx
+      INTEGER(C_INT64_T), VALUE       :: ncell
+      REAL(C_DOUBLE),     INTENT(IN)  :: conc(*), rconst(*)
+      REAL(C_DOUBLE),     INTENT(OUT) :: aout(*)
+    END FUNCTION
+    ! One cell's KPP-Standalone state file (KppSa_Write_Samples); path
+    ! NUL-terminated; names: NSPEC records of GCKPP_NAME_LEN characters
+    ! (blank-padded), C(NSPEC), ATOL(NSPEC), RCONST(NREACT), AOUT(NREACT)
+    INTEGER(C_INT) FUNCTION gckpp_kppsa_write(mech_id, path, meta, names, conc, atol, rconst, &
+         aout) BIND(C, name="gckpp_kppsa_write")
+      IMPORT :: C_INT, C_CHAR, C_DOUBLE, gckpp_kppsa_meta
+      INTEGER(C_INT),         VALUE      :: mech_id
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*), names(*)
+      TYPE(gckpp_kppsa_meta), INTENT(IN) :: meta
+      REAL(C_DOUBLE),         INTENT(IN) :: conc(*), atol(*), rconst(*), aout(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION gckpp_kppsa_read(mech_id, path, meta, names, conc, atol, rconst, &
+         aout) BIND(C, name="gckpp_kppsa_read")
+      IMPORT :: C_INT, C_CHAR, C_DOUBLE, gckpp_kppsa_meta
+      INTEGER(C_INT),         VALUE       :: mech_id
+      CHARACTER(KIND=C_CHAR), INTENT(IN)  :: path(*)
+      TYPE(gckpp_kppsa_meta), INTENT(OUT) :: meta
+      CHARACTER(KIND=C_CHAR), INTENT(OUT) :: names(*)
+      REAL(C_DOUBLE),         INTENT(OUT) :: conc(*), atol(*), rconst(*), aout(*)
+    END FUNCTION
+    ! KPPHvalue(nx,ny,nz) as the restart variable Chem_KPPHvalue (netCDF
+    ! classic); path and model_tag NUL-terminated
+    INTEGER(C_INT) FUNCTION gckpp_kpph_restart_write(path, nx, ny, nz, kpph, model_tag) &
+         BIND(C, name="gckpp_kpph_restart_write")
+      IMPORT :: C_INT, C_INT64_T, C_CHAR, C_DOUBLE
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*), model_tag(*)
+      INTEGER(C_INT64_T),     VALUE      :: nx, ny, nz
+      REAL(C_DOUBLE),         INTENT(IN) :: kpph(*)
+    END FUNCTION
+    ! found = 0 and KPPHvalue = 0 when the file has no Chem_KPPHvalue
+    INTEGER(C_INT) FUNCTION gckpp_kpph_restart_read(path, nx, ny, nz, kpph, found, model_tag) &
+         BIND(C, name="gckpp_kpph_restart_read")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_CHAR, C_DOUBLE
+      CHARACTER(KIND=C_CHAR), INTENT(IN)  :: path(*)
+      INTEGER(C_INT64_T),     VALUE       :: nx, ny, nz
+      REAL(C_DOUBLE),         INTENT(OUT) :: kpph(*)
+      INTEGER(C_INT32_T),     INTENT(OUT) :: found
+      CHARACTER(KIND=C_CHAR), INTENT(OUT) :: model_tag(*)
     END FUNCTION
     TYPE(C_PTR) FUNCTION gckpp_last_error() BIND(C, name="gckpp_last_error")
       IMPORT :: C_PTR

@@ -1,0 +1,168 @@
+"""Do_FullChem's and ChemMercury's per-cell protocol on the Fortran side:
+geos-chem_amd/fortran/chem_host (flang, include/gckpp_hip_mod.F90 and
+geos-chem_amd/fortran/gckpp_chem_mod.F90) reads KPPHvalue from a restart
+file, runs one chemistry step of a gathered 12x10x5 grid with forced
+retries, and writes the cells, the KppDiags fields, the next KPPHvalue as a
+restart file and one active cell's KPP-Standalone state file.
+
+Expected, built here from the device calls of the Python binding:
+  first Integrate with Hstart = KPPHvalue; KppDiags = its ISTATUS(1:8) and
+  its negative count; the failed cells again from their original C with
+  Hstart = 0, adding ISTATUS(1:7) (fullchem, GeosCore/fullchem_mod.F90:
+  1079-1203) or ISTATUS(1:8) and the negative count (Hg, Hg_UpdateKppDiags,
+  GeosCore/mercury_mod.F90:4622-4719); cells that fail twice keep their
+  concentrations and KPPHvalue (Failed2x, :1204-1245), the others take
+  RSTATUS(3) and C = MAX(C, 0).  Everything bitwise; the successful cells
+  also bitwise the device step gckpp_fullchem_step_dev; the state file
+  holds the cell's state before Integrate bitwise (its Hexit to the four
+  decimals of its F11.4 field)."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+HOST = ROOT / "geos-chem_amd" / "fortran" / "chem_host"
+NX, NY, NZ = 12, 10, 5
+
+
+@pytest.fixture(scope="module")
+def T():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _write_batch(path, m, w, solver, active, max_cells, dt, ic, rc):
+    sc = m.species_consts
+    with open(path, "wb") as f:
+        f.write(np.array([m.mech_id, solver, 0, active], dtype=np.int32).tobytes())
+        f.write(np.array([NX, NY, NZ, max_cells, w.het.shape[1]], dtype=np.int64).tobytes())
+        f.write(np.array([dt], dtype=np.float64).tobytes())
+        f.write(np.asarray(ic, dtype=np.int32).tobytes())
+        f.write(np.asarray(rc, dtype=np.float64).tobytes())
+        for a in (m.atol(), m.rtol(), sc["mw"], sc["henry_k0"], sc["henry_cr"]):
+            f.write(np.asarray(a, dtype=np.float64).tobytes())
+        for a in (w.met, w.photol, w.het, w.conc):
+            f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        f.write(b"".join(s.encode().ljust(64) for s in m.species))
+
+
+def _read_result(path, m, n):
+    b = open(path, "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(b, dtype=dtype, count=count, offset=off)
+        off += a.nbytes
+        return a
+    out = dict(rcode=int(take(np.int32, 1)[0]), nfail=int(take(np.int64, 1)[0]))
+    out["conc"] = take(np.float64, m.nspec * n).reshape(m.nspec, n)
+    out["kpph"] = take(np.float64, n)
+    out["ierr"] = take(np.int32, n)
+    out["diags"] = take(np.float32, 9 * n).reshape(9, n)
+    out["found"] = int(take(np.int32, 1)[0])
+    out["tag"] = b[off:off + 64].decode().strip()
+    assert off + 64 == len(b)
+    return out
+
+
+@pytest.mark.parametrize("name,solver_name", [("fullchem", "persistent"), ("fullchem", "plain"),
+                                              ("Hg", "persistent")])
+def test_chem_host_protocol_diags_restart_and_state_file(name, solver_name, mech_of, T, tmp_path):
+    print("case: Fortran chem host, %s, %s solver, 600 cells, forced retries" % (name, solver_name),
+          flush=True)
+    from gckpp_amd import hip, options, synth
+    assert HOST.exists(), "the Fortran host is built with the library (make -C geos-chem_amd/csrc)"
+    m = mech_of(name)
+    solver = {"plain": hip.SOLVER_PLAIN, "persistent": hip.SOLVER_PERSISTENT}[solver_name]
+    n = NX * NY * NZ
+    w = synth.box_cells(m, n, 29)
+    ic, rc = options.icntrl_for(name)
+    dt = 1200.0
+    dev = lambda a: T.from_numpy(np.ascontiguousarray(a)).cuda()
+
+    # cold-start step counts, to set a step limit that fails many first calls
+    ctx = hip.Context(m, n)
+    ctx.set_solver(solver)
+    rct = ctx.rates(dev(w.met), dev(w.photol), dev(w.het), dev(w.conc))
+    c = dev(w.conc)
+    ist, _, _ = ctx.integrate(c, rct, T.zeros(n, dtype=T.float64, device="cuda"), 0.0, dt, ic, rc)
+    ic = np.array(ic, dtype=np.int32)
+    ic[3] = int(np.median(ist[2].cpu().numpy())) + 2
+    kpph0 = np.full(n, 1e-13)
+    kpph0[::7] = 30.0
+
+    # expected: the protocol through the device calls
+    conc0 = dev(w.conc)
+    c1 = conc0.clone()
+    ist1, rst1, e1 = ctx.integrate(c1, rct, dev(kpph0), 0.0, dt, ic, rc)
+    c1, ist1, rst1, e1 = (x.cpu().numpy() for x in (c1, ist1, rst1, e1))
+    diags = np.zeros((9, n), dtype=np.float32)
+    diags[:8] = ist1[:8]
+    diags[8] = (c1 < 0).sum(0)
+    fail = np.nonzero(e1 < 0)[0]
+    rc2 = np.array(rc, dtype=np.float64)
+    rc2[2] = 0.0
+    c2 = dev(w.conc[:, fail])
+    ist2, rst2, e2 = ctx.integrate(c2, rct[:, fail].contiguous(),
+                                   T.zeros(len(fail), dtype=T.float64, device="cuda"), 0.0, dt, ic, rc2)
+    c2, ist2, rst2, e2 = (x.cpu().numpy() for x in (c2, ist2, rst2, e2))
+    cf, ef, rstf, istf = c1.copy(), e1.copy(), rst1.copy(), ist1.copy()
+    cf[:, fail], ef[fail], rstf[:, fail], istf[:, fail] = c2, e2, rst2, ist2
+    k = 8 if name == "Hg" else 7
+    diags[:k, fail] += ist2[:k].astype(np.float32)
+    if name == "Hg":
+        diags[8, fail] += (c2 < 0).sum(0).astype(np.float32)
+    twice = ef < 0
+    kpph = np.where(twice, kpph0, rstf[2])
+    cf = np.where(twice[None, :], w.conc, np.maximum(cf, 0.0))
+    print("  first-call failures %d, failed twice %d" % (len(fail), int(twice.sum())), flush=True)
+    assert len(fail) > 20 and 0 < twice.sum() < len(fail)
+    aout = ctx.aout(conc0, rct).cpu().numpy()
+    rct_h = rct.cpu().numpy()
+
+    # the device step on the same cells agrees where the cells did not fail twice
+    c3 = dev(w.conc)
+    hnew3, ist3, e3, ok3 = ctx.fullchem_step(dev(w.met), dev(w.photol), dev(w.het), c3, dev(kpph0),
+                                             dt, ic, rc)
+    c3, hnew3, e3 = c3.cpu().numpy(), hnew3.cpu().numpy(), e3.cpu().numpy()
+    assert np.array_equal(e3, ef)
+    assert np.array_equal(c3[:, ~twice], cf[:, ~twice]) and np.array_equal(hnew3[~twice], kpph[~twice])
+    ctx.close()
+
+    # the Fortran host
+    active = int(np.nonzero(~twice & (np.arange(n) % 7 == 0))[0][0])
+    rin, rout = tmp_path / "restart_in.nc", tmp_path / "restart_out.nc"
+    hip.kpph_restart_write(rin, kpph0.reshape((NX, NY, NZ), order="F"), model_tag="test-model")
+    fin, fres, fsa = tmp_path / "batch.bin", tmp_path / "result.bin", tmp_path / "active.txt"
+    _write_batch(fin, m, w, solver, active + 1, 256, dt, ic, rc)
+    r = subprocess.run([str(HOST), str(fin), str(rin), str(rout), str(fres), str(fsa)],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    got = _read_result(fres, m, n)
+    assert got["rcode"] == -1 and got["nfail"] == int(twice.sum())
+    assert got["found"] == 1 and got["tag"] == "test-model"
+    assert np.array_equal(got["ierr"], ef)
+    assert np.array_equal(got["conc"], cf)
+    assert np.array_equal(got["kpph"], kpph)
+    assert np.array_equal(got["diags"], diags)
+    # the restart it wrote
+    k2, found, tag = hip.kpph_restart_read(rout, (NX, NY, NZ))
+    assert found and tag == "chem_host" and np.array_equal(k2.reshape(-1, order="F"), kpph)
+    # the active cell's KPP-Standalone file
+    meta, names, sc, sa, sr, so = hip.kppsa_read(m, fsa)
+    assert names == list(m.species)
+    assert np.array_equal(sc, w.conc[:, active]) and np.array_equal(sr, rct_h[:, active])
+    assert np.array_equal(so, aout[:, active])
+    assert np.array_equal(sa[:m.nvar], m.atol())
+    assert meta.h_init == kpph0[active] and meta.rcntrl[2] == kpph0[active]
+    assert abs(meta.h_exit - rstf[1, active]) <= 5e-5          # written as F11.4
+    assert meta.nsteps == istf[2, active] and meta.dt == dt
+    assert list(meta.icntrl) == [int(x) for x in ic]
