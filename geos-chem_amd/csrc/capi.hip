@@ -7,6 +7,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/gckpp_hip.h"
@@ -89,6 +91,7 @@ struct gckpp_ctx {
   double *hwork;         // scratch of the resume launch, hwaves waves of hw_fields*64
   int64_t hwaves, hw_fields;
   bool ho_timed;         // ev_ho recorded by a launch
+  int64_t na_waves;      // resident waves of the cell-per-wave kernel (no-allocation entry)
 };
 
 extern "C" const char *gckpp_last_error(void) { return g_err; }
@@ -142,6 +145,35 @@ static int set_pconfig(gckpp_ctx *c, int variant, int wpb) {
   return 0;
 }
 
+// Scratch of gckpp_integrate_noalloc_dev: the work-stealing counter
+// (the first 256 bytes) and the blocks of up to kNoAllocWaves waves of the
+// cell-per-wave solver (each wave's rate constants, reaction products,
+// Fcn0 and stage vectors: wave_scratch_fields(mech) fields of 64 doubles,
+// 73 for fullchem, the largest).  768 waves are three fullchem cells per
+// CU on 256 CUs; 28 MiB per device, held until the process ends.  Host
+// code reaches each device's copy through the address that
+// hipGetSymbolAddress gives with that device current (g_na_ptr), never
+// through the array's name.
+constexpr int64_t kNoAllocWaves = 768, kNoAllocFields = 73, kNoAllocHead = 32;
+constexpr int64_t kNoAllocDoubles = kNoAllocHead + kNoAllocWaves * kNoAllocFields * 64;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_na_mu;
+static double *g_na_ptr[kMaxDevices];
+
+// Once per device, with the device current: the address of its scratch.
+static int noalloc_setup(int dev) {
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (dev < 0 || dev >= kMaxDevices) return fail("gckpp_init: device %d beyond %d", dev, kMaxDevices);
+  if (g_na_ptr[dev]) return 0;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail("gckpp_init: no address for the static scratch: %s", hipGetErrorString(e));
+  g_na_ptr[dev] = (double *)p;
+  return 0;
+}
+
 // A failed gckpp_init releases what it allocated before reporting.
 #define INIT_OK(x)                                                             \
   do {                                                                         \
@@ -161,6 +193,12 @@ extern "C" int gckpp_init(int mech_id, int64_t max_cells, gckpp_ctx **out) {
   c->mech = mech_id;
   c->info = mech_info(mech_id);
   INIT_OK(hipGetDevice(&c->device));
+  if (noalloc_setup(c->device) != 0) {
+    const std::string why = g_err;
+    gckpp_finalize(c);
+    return fail("%s", why.c_str());
+  }
+  c->na_waves = persist_resident_waves(mech_id, PERSIST_WAVE_VARIANT, 1);
   c->max_cells = max_cells;
   c->wstride = (max_cells + 63) / 64 * 64;
   int64_t per_cell = integrate_work_per_cell(c->info);
@@ -485,6 +523,69 @@ extern "C" int gckpp_integrate_dev(gckpp_ctx *c, int64_t ncell, double tin, doub
                                    void *stream) {
   return integrate_impl(c, ncell, tin, tout, icntrl, rcntrl, hstart, conc, rconst, istatus,
                         rstatus, ierr, nullptr, nullptr, stream);
+}
+
+// Integrate without allocating: the cell-per-wave solver with its scratch
+// in the static array above (see the header).  Everything it touches is
+// the caller's arrays, the context's tolerances and that array.
+extern "C" int gckpp_integrate_noalloc_dev(gckpp_ctx *c, int64_t ncell, double tin, double tout,
+                                           const int32_t icntrl[20], const double rcntrl[20],
+                                           const double *hstart, double *conc,
+                                           const double *rconst, int32_t *istatus,
+                                           double *rstatus, int32_t *ierr, void *stream) {
+  if (!c) return fail("gckpp_integrate_noalloc_dev: null context");
+  if (ncell < 0) return fail("gckpp_integrate_noalloc_dev: negative ncell");
+  if (ncell == 0) return 0;
+  if (!conc || !rconst) return fail("gckpp_integrate_noalloc_dev: null conc or rconst");
+  double *ptr = (c->device >= 0 && c->device < kMaxDevices) ? g_na_ptr[c->device] : nullptr;
+  if (!ptr) return fail("gckpp_integrate_noalloc_dev: no static scratch on device %d", c->device);
+  const int64_t fields = wave_scratch_fields(c->mech);
+  const int64_t cap = (kNoAllocDoubles - kNoAllocHead) / (64 * fields);
+  if (cap < 1)
+    return fail("gckpp_integrate_noalloc_dev: one wave needs %lld doubles of scratch, the static "
+                "scratch holds %lld", (long long)(64 * fields), (long long)(kNoAllocDoubles - kNoAllocHead));
+  if (c->pwaves_max > cap)
+    return fail("gckpp_integrate_noalloc_dev: the context's wave cap (%lld waves, "
+                "gckpp_set_persistent_waves) exceeds the %lld waves (%.1f MB) the static scratch "
+                "holds for this mechanism", (long long)c->pwaves_max, (long long)cap,
+                kNoAllocDoubles * 8.0 / 1e6);
+  hipStream_t s = (hipStream_t)stream;
+  RosParams p;
+  char why[256] = "";
+  const int code = resolve_options(c, tin, tout, icntrl, rcntrl, &p, why, sizeof why);
+  if (code == -100) return fail("gckpp_integrate_noalloc_dev: %s", why);
+  if (code < 0) {
+    hipLaunchKernelGGL(k_fill_status, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s,
+                       ncell, ierr, code, istatus, rstatus);
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
+  int64_t nw = c->na_waves > 0 ? c->na_waves : 1;
+  if (nw > cap) nw = cap;
+  if (c->pwaves_max > 0 && nw > c->pwaves_max) nw = c->pwaves_max;
+  if (nw > ncell) nw = ncell;
+  PersistArgs pa{};
+  IntegrateArgs &a = pa.q[0];
+  a.ncell = ncell;
+  a.conc = conc;
+  a.cstride = ncell;
+  a.rconst = rconst;
+  a.rstride = ncell;
+  a.hstart = hstart;
+  a.atol = c->atol;
+  a.rtol = c->rtol;
+  a.prm = p;
+  a.istatus = istatus;
+  a.rstatus = rstatus;
+  a.ierr = ierr;
+  a.sstride = ncell;
+  pa.qend[0] = pa.qend[1] = pa.qend[2] = ncell;
+  pa.work = ptr + kNoAllocHead;
+  pa.slot_fields = fields;
+  pa.counter = (unsigned int *)ptr;   // zeroed by a kernel in launch_persist
+  if (launch_persist(c->mech, PERSIST_WAVE_VARIANT, 1, pa, nw, s) != 0)
+    return fail("gckpp_integrate_noalloc_dev: launch failed: %s", hipGetErrorString(hipGetLastError()));
+  return 0;
 }
 
 static int comp(gckpp_ctx *c, int which, int64_t ncell, const double *in, const double *rct,

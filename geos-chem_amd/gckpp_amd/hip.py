@@ -38,7 +38,7 @@ SYMBOLS = [
     "gckpp_batch_handoff_count",
     "gckpp_persistent_handoff_stats",
     "gckpp_aout_dev", "gckpp_aout", "gckpp_kppsa_write", "gckpp_kppsa_read",
-    "gckpp_kpph_restart_write", "gckpp_kpph_restart_read",
+    "gckpp_kpph_restart_write", "gckpp_kpph_restart_read", "gckpp_integrate_noalloc_dev",
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
@@ -100,6 +100,8 @@ def load():
                                       dp, i32p, i32p]
     L.gckpp_integrate_dev.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double, i32p, dp,
                                       dp, dp, dp, i32p, dp, i32p, vp]
+    L.gckpp_integrate_noalloc_dev.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double, i32p, dp,
+                                              dp, dp, dp, i32p, dp, i32p, vp]
     L.gckpp_integrate.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double, i32p, dp,
                                   dp, dp, dp, i32p, dp, i32p]
     L.gckpp_fullchem_step_dev.argtypes = [vp, i64, ctypes.c_double, i32p, dp, dp, dp, dp, i64,

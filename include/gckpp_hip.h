@@ -90,6 +90,24 @@ int gckpp_integrate_dev(gckpp_ctx *ctx, int64_t ncell, double tin, double tout,
                         const double *hstart, double *conc, const double *rconst,
                         int32_t *istatus, double *rstatus, int32_t *ierr,
                         void *stream);
+/* gckpp_integrate_dev without any allocation (no reference counterpart:
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops).  Same arguments and results (bitwise) as
+ * gckpp_integrate_dev with the persistent solver's cell-per-wave variant;
+ * any ncell.  Its scratch is a static device array of 28 MiB per device
+ * (the work-stealing counter and up to 768 fullchem waves), reserved when
+ * the library is loaded and held until the process ends; the call itself
+ * makes no hipMalloc or other allocation.  Fails with a message when the
+ * context's wave cap (gckpp_set_persistent_waves) asks for more waves than
+ * the scratch holds.  All calls on one device share the scratch: calls
+ * that overlap on one device (concurrent streams or threads) race.  The
+ * cell order and step log of gckpp_set_cell_order / gckpp_set_step_log do
+ * not apply. */
+int gckpp_integrate_noalloc_dev(gckpp_ctx *ctx, int64_t ncell, double tin, double tout,
+                                const int32_t icntrl[20], const double rcntrl[20],
+                                const double *hstart, double *conc, const double *rconst,
+                                int32_t *istatus, double *rstatus, int32_t *ierr,
+                                void *stream);
 /* The same with host arrays, synchronous; any ncell, in chunks of at most
  * max_cells through the staging buffers of gckpp_rates (allocated once);
  * istatus, rstatus and ierr may be null (not returned).  The cell order and
