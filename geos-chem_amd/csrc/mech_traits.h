@@ -15,8 +15,8 @@ namespace gckpp {
 // flag bits (eight blocks a word), per round and lane a head word; nb
 // blocks, a multiple of eight.
 struct WProg {
-  const uint4 *term;
-  const unsigned *flag, *head;
+  const uint4 *ops;
+  const unsigned *flag;
   int nb;
 };
 enum { WP_FV = 0, WP_JV = 1, WP_LU = 2, WP_SF = 3, WP_SB = 4 };
@@ -77,27 +77,29 @@ enum { WP_FV = 0, WP_JV = 1, WP_LU = 2, WP_SF = 3, WP_SB = 4 };
     static constexpr int NSF_BATCH = mech_##NS::NSF_BATCH;                     \
     static constexpr int NSB_BATCH = mech_##NS::NSB_BATCH;                     \
     static constexpr int WV_ZERO = mech_##NS::WV_ZERO, WV_X0 = mech_##NS::WV_X0; \
+    static constexpr int WV_ONE = mech_##NS::WV_ONE, WV_SINK = mech_##NS::WV_SINK; \
     static constexpr int WV_Y0 = mech_##NS::WV_Y0, WV_SH = mech_##NS::WV_SH;   \
+    static constexpr int WV_NCOEF = mech_##NS::WV_NCOEF;                       \
     static constexpr int WV_G_AB = mech_##NS::WV_G_AB, WV_NAB = mech_##NS::WV_NAB; \
     static constexpr int WV_G_F0 = mech_##NS::WV_G_F0, WV_G_K = mech_##NS::WV_G_K; \
     static constexpr int WV_G_SIZE = mech_##NS::WV_G_SIZE;                     \
     static __device__ const double *wv_coef() { return mech_##NS::wv_coef; }   \
     static __device__ WProg wprog(int p) {                                     \
       switch (p) {                                                             \
-        case WP_FV: return WProg{(const uint4 *)mech_##NS::wv_fv_term,         \
-                                 mech_##NS::wv_fv_flag, mech_##NS::wv_fv_head, \
+        case WP_FV: return WProg{(const uint4 *)mech_##NS::wv_fv_ops,         \
+                                 mech_##NS::wv_fv_flag, \
                                  mech_##NS::WV_FV_NB};                         \
-        case WP_JV: return WProg{(const uint4 *)mech_##NS::wv_jv_term,         \
-                                 mech_##NS::wv_jv_flag, mech_##NS::wv_jv_head, \
+        case WP_JV: return WProg{(const uint4 *)mech_##NS::wv_jv_ops,         \
+                                 mech_##NS::wv_jv_flag, \
                                  mech_##NS::WV_JV_NB};                         \
-        case WP_LU: return WProg{(const uint4 *)mech_##NS::wv_lu_term,         \
-                                 mech_##NS::wv_lu_flag, mech_##NS::wv_lu_head, \
+        case WP_LU: return WProg{(const uint4 *)mech_##NS::wv_lu_ops,         \
+                                 mech_##NS::wv_lu_flag, \
                                  mech_##NS::WV_LU_NB};                         \
-        case WP_SF: return WProg{(const uint4 *)mech_##NS::wv_sf_term,         \
-                                 mech_##NS::wv_sf_flag, mech_##NS::wv_sf_head, \
+        case WP_SF: return WProg{(const uint4 *)mech_##NS::wv_sf_ops,         \
+                                 mech_##NS::wv_sf_flag, \
                                  mech_##NS::WV_SF_NB};                         \
-        default: return WProg{(const uint4 *)mech_##NS::wv_sb_term,            \
-                              mech_##NS::wv_sb_flag, mech_##NS::wv_sb_head,    \
+        default: return WProg{(const uint4 *)mech_##NS::wv_sb_ops,            \
+                              mech_##NS::wv_sb_flag,    \
                               mech_##NS::WV_SB_NB};                            \
       }                                                                        \
     }                                                                          \

@@ -26,8 +26,8 @@
 namespace gckpp {
 namespace {
 
-constexpr unsigned kIdle = 0xFFFFu;
-constexpr int kPF = 8;  // blocks of term words loaded ahead (wave_sched.PF)
+constexpr int kPF = 8;  // blocks of program words loaded ahead (wave_sched.PF)
+constexpr unsigned F_START = 1u, F_END = 2u, F_LEVEL = 4u, F_DIV = 8u;  // block flags
 
 // Orders the wave's LDS accesses across lanes: LDS instructions of one
 // wavefront execute in program order, so it suffices that the compiler moves
@@ -53,100 +53,121 @@ __device__ __forceinline__ double ld(const char *base, unsigned off) {
 __device__ __forceinline__ void ld_st(double *sh, unsigned off, double v) {
   *(double *)((char *)sh + off) = v;
 }
+// A double at byte offset off of a buffer (no address arithmetic: the
+// offset goes straight into the load).
+__device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
+}
 
-// Runs one wave program.  MODE 0 (Vdot, Jacobian): operand a from the
-// reaction scratch gab, b from the coefficient table coef (byte offsets);
-// a chunk's sum starts at +0.0 and a step is acc = fma(b, a, acc).  MODE 1
-// (LU, substitution): both operands in LDS (shb); the sum starts at the
-// destination slot and a step is acc = fma(-a, b, acc); DIV: a round's
-// result is divided by its divisor slot when it names one.  store(dest,
-// hi, acc) writes a round's result (byte offsets).
+// Runs one wave program (wave_sched.encode).  MODE 0 (Vdot, Jacobian):
+// operand a from the reaction scratch (buffer rab, PADA the byte offset of
+// its +0.0 slot), b from the coefficient table (buffer rcf), byte offsets;
+// a round's sum starts at +0.0 and a step is acc = fma(b, a, acc).  MODE 1
+// (LU, substitutions): both operands in LDS (shb); the sum starts at
+// LDS[dest] and a step is acc = fma(-a, b, acc); DIV: a round flagged F_DIV
+// divides its result by LDS[divisor] (the +1.0 slot for lanes that do not
+// divide).  store(dest, x, v) writes a lane's value (x: the header's
+// second word); it runs after every block, so the destination holds the
+// partial sum until the round's last block stores the result -- no other
+// chunk of the level reads it.
 //
-// One flat loop over blocks of four steps.  The term words of the next kPF
-// blocks are in flight while a block runs, the head words of the next
-// three rounds likewise.  A block's flag bits (uniform) say where rounds
-// start and end and where a level ends.  The operands of the next block --
-// and, when it starts a round, its starting value and divisor -- are
-// requested before the current block's FMAs unless the current block ends
-// a level: within a level no chunk reads what another chunk of the level
-// stores, so those reads may pass the level's stores; the next level's
-// reads are issued after the ordering point.
-template <int MODE, bool DIV, int ZB, class Store>
-__device__ __forceinline__ void run_prog(char *shb, const WProg &p, int lane, const char *gab,
-                                         const char *coef, Store store) {
-  const uint4 *tp = p.term + lane;
-  uint4 ring[kPF];
+// One flat loop over blocks of four ops, unrolled by kPF: the program
+// words of the next kPF blocks are in flight while a block runs, and the
+// operands of the next D blocks.  A round's first block loads its header's
+// two slots like a term's operands (MODE 1: the starting value and the
+// divisor).  Every block issues the same loads in the same order, so the
+// waits for them are counted exactly.  Within a level no chunk reads what
+// another chunk of the level stores, so those reads may pass the level's
+// stores; in MODE 1 the next block's operands are requested after the
+// ordering point that ends a level.
+template <int MODE, bool DIV, int D, unsigned PADA, class Store>
+__device__ __forceinline__ void run_prog(char *shb, const WProg &p, int lane,
+                                         __amdgpu_buffer_rsrc_t rab, __amdgpu_buffer_rsrc_t rcf,
+                                         Store store) {
+  constexpr int R = D + 1;  // operand slots; divides kPF
+  static_assert(kPF % R == 0, "operand ring");
+  static_assert(MODE == 0 || D == 1, "LDS operands are requested one block ahead");
+  const uint4 *tp = p.ops + lane;
+  uint4 rw[kPF][2];
 #pragma unroll
-  for (int i = 0; i < kPF; ++i) ring[i] = tp[i * 64];
-  const unsigned *hp = p.head + lane;
-  unsigned h0 = hp[0], h1 = hp[64], h2 = hp[128];
-  double opa[2][4], opb[2][4], init[2], divv[2], dv = 1.0;
-  // requests block operands (words w), and a round start's value/divisor
-  auto fetch = [&](int slot, const uint4 &w, unsigned f, unsigned h) {
-    const unsigned ww[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const unsigned ao = ww[u] & 0xFFFFu, bo = ww[u] >> 16;
-      if constexpr (MODE == 1) {
-        opa[slot][u] = ld(shb, ao);
-        opb[slot][u] = ld(shb, bo);
-      } else {
-        opa[slot][u] = ld(gab, ao);
-        opb[slot][u] = ld(coef, bo);
-      }
-    }
+  for (int i = 0; i < kPF; ++i) {
+    rw[i][0] = tp[(2 * i) * 64];
+    rw[i][1] = tp[(2 * i + 1) * 64];
+  }
+  double oa[R][4], ob[R][4];
+  unsigned hd[R], hx[R];
+  auto opa = [&](unsigned off) -> double { return MODE == 1 ? ld(shb, off) : bld(rab, off); };
+  auto opb = [&](unsigned off) -> double { return MODE == 1 ? ld(shb, off) : bld(rcf, off); };
+  // requests the operands of a block (words w0 w1, flags f) into slot sl
+  auto fetch = [&](int sl, const uint4 &w0, const uint4 &w1, unsigned f) {
+    const bool st = (f & F_START) != 0;
+    hd[sl] = w0.x;
+    hx[sl] = w0.y;
     if constexpr (MODE == 1) {
-      if (f & 1u) {
-        const unsigned d = h & 0xFFFFu;
-        init[slot] = ld(shb, d != kIdle ? d : (unsigned)ZB);
-        if constexpr (DIV) {
-          const unsigned v = h >> 16;
-          divv[slot] = ld(shb, v != kIdle ? v : (unsigned)ZB);
-        }
-      }
+      oa[sl][0] = ld(shb, w0.x);
+      ob[sl][0] = ld(shb, w0.y);
+    } else {
+      oa[sl][0] = bld(rab, st ? PADA : w0.x);
+      ob[sl][0] = bld(rcf, st ? 0u : w0.y);
     }
+    oa[sl][1] = opa(w0.z);
+    ob[sl][1] = opb(w0.w);
+    oa[sl][2] = opa(w1.x);
+    ob[sl][2] = opb(w1.y);
+    oa[sl][3] = opa(w1.z);
+    ob[sl][3] = opb(w1.w);
   };
-  int r = 0;
-  double acc = 0.0;
+  double acc = 0.0, dv = 1.0;
+  unsigned dest = 0, xw = 0;
+  auto term = [&](double a, double b) { return MODE == 0 ? fma(b, a, acc) : fma(-a, b, acc); };
+  // runs the block in slot sl (flags f)
+  auto run = [&](int sl, unsigned f) {
+    const bool st = (f & F_START) != 0;
+    const double a0 = oa[sl][0], b0 = ob[sl][0];
+    const double t0 = term(a0, b0);
+    acc = st ? (MODE == 1 ? a0 : 0.0) : t0;
+    if constexpr (DIV) dv = st ? b0 : dv;
+    dest = st ? hd[sl] : dest;
+    xw = st ? hx[sl] : xw;
+    acc = term(oa[sl][1], ob[sl][1]);
+    acc = term(oa[sl][2], ob[sl][2]);
+    acc = term(oa[sl][3], ob[sl][3]);
+    double v = acc;
+    if constexpr (DIV) {
+      if ((f & (F_END | F_DIV)) == (F_END | F_DIV)) v = v / dv;
+    }
+    store(dest, xw, v);
+  };
   unsigned fl = p.flag[0];
-  fetch(0, ring[0], fl & 15u, h0);
+  // blocks 0..D-1; the ring then holds blocks D..kPF+D-1
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    fetch(j, rw[j][0], rw[j][1], (fl >> (4 * j)) & 15u);
+    rw[j][0] = tp[(2 * (kPF + j)) * 64];
+    rw[j][1] = tp[(2 * (kPF + j) + 1) * 64];
+  }
 #pragma unroll 1
   for (int k0 = 0; k0 < p.nb; k0 += kPF) {
-    const unsigned fln = p.flag[(k0 + kPF) >> 3];
+    const unsigned fln = p.flag[(k0 >> 3) + 1];
+    const uint4 *tq = tp + (2 * (k0 + kPF)) * 64;  // program words of block k0 + kPF
 #pragma unroll
     for (int i = 0; i < kPF; ++i) {
-      const int cur = i & 1, nxt = cur ^ 1;
       const unsigned f = (fl >> (4 * i)) & 15u;
-      const unsigned fn = i + 1 < kPF ? (fl >> (4 * (i + 1))) & 15u : fln & 15u;
-      const unsigned hn = (f & 2u) ? h1 : h0;  // head of the next block's round
-      const uint4 wn = ring[(i + 1) % kPF];
-      if (!(f & 4u)) fetch(nxt, wn, fn, hn);
-      ring[i] = tp[(k0 + i + kPF) * 64];
-      if (f & 1u) {
-        acc = MODE == 1 ? init[cur] : 0.0;
-        if constexpr (DIV) dv = divv[cur];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        acc = MODE == 0 ? fma(opb[cur][u], opa[cur][u], acc) : fma(-opa[cur][u], opb[cur][u], acc);
-      if (f & 2u) {
-        const unsigned dest = h0 & 0xFFFFu;
-        if (dest != kIdle) {
-          double v = acc;
-          if constexpr (DIV) {
-            if ((h0 >> 16) != kIdle) v = v / dv;
-          }
-          store(dest, h0 >> 16, v);
-        }
-        h0 = h1;
-        h1 = h2;
-        h2 = hp[(r + 3) * 64];
-        ++r;
-      }
-      if (f & 4u) {
+      const int jn = i + D;  // the block whose operands are requested now, relative to k0
+      const unsigned fj = jn < kPF ? (fl >> (4 * jn)) & 15u : (fln >> (4 * (jn - kPF))) & 15u;
+      // MODE 1 (D = 1): the next block's operands wait for a level's end
+      const bool hold = MODE == 1 && (f & F_LEVEL);
+      if (!hold) fetch(jn % R, rw[jn % kPF][0], rw[jn % kPF][1], fj);
+      run(i % R, f);
+      if (hold) {
         wave_sync();
-        fetch(nxt, wn, fn, hn);
+        fetch(jn % R, rw[jn % kPF][0], rw[jn % kPF][1], fj);
       }
+      rw[jn % kPF][0] = tq[(2 * jn) * 64];
+      rw[jn % kPF][1] = tq[(2 * jn + 1) * 64];
     }
     fl = fln;
   }
@@ -174,6 +195,10 @@ struct WaveCell {
   __device__ double *ab() const { return g + M::WV_G_AB; }
   __device__ double *F0() const { return g + M::WV_G_F0; }
   __device__ double *K(int s) const { return g + M::WV_G_K + s * M::NVAR; }
+  // the reaction scratch (with its +0.0 pad slot) and the coefficient
+  // table as buffers of the Vdot and Jacobian programs' operands
+  __device__ __amdgpu_buffer_rsrc_t rab() const { return brsrc(ab(), 8 * (M::WV_NAB + 1)); }
+  __device__ __amdgpu_buffer_rsrc_t rcf() const { return brsrc(M::wv_coef(), 8 * M::WV_NCOEF); }
 };
 
 // Fun: A(r) = RCT(r)*prod conc(reactants) into the scratch, then Vdot into
@@ -196,9 +221,8 @@ __device__ void wave_fun(const WaveCell<M> &c, int in0) {
     ab[r] = v;
   }
   wave_sync_global();
-  run_prog<0, false, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_FV), c.lane, (const char *)ab,
-                              (const char *)M::wv_coef(),
-                              [&](unsigned d, unsigned, double acc) { ld_st(sh, d, acc); });
+  run_prog<0, false, 3, 8u * M::WV_NAB>((char *)sh, M::wprog(WP_FV), c.lane, c.rab(), c.rcf(),
+                                        [&](unsigned d, unsigned, double acc) { ld_st(sh, d, acc); });
 }
 // Ghimj = sign*JVS + diag on the diagonal into the LU block (partials from Y
 // into the scratch first); returns IER of KppDecomp's pivot check.
@@ -220,9 +244,8 @@ __device__ int wave_jac(const WaveCell<M> &c, double sign, double diag) {
     ab[p] = v;
   }
   wave_sync_global();
-  run_prog<0, false, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_JV), c.lane, (const char *)ab,
-                              (const char *)M::wv_coef(),
-                              [&](unsigned d, unsigned isdiag, double acc) {
+  run_prog<0, false, 3, 8u * M::WV_NAB>((char *)sh, M::wprog(WP_JV), c.lane, c.rab(), c.rcf(),
+                        [&](unsigned d, unsigned isdiag, double acc) {
                                 double v = sign * acc;
                                 if (isdiag) v = v + diag;
                                 ld_st(sh, d, v);
@@ -242,24 +265,26 @@ __device__ int wave_jac(const WaveCell<M> &c, double sign, double diag) {
 template <class M>
 __device__ void wave_decomp(const WaveCell<M> &c) {
   double *sh = c.sh;
-  run_prog<1, true, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_LU), c.lane, nullptr, nullptr,
-                                    [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
+  run_prog<1, true, 1, 0u>((char *)sh, M::wprog(WP_LU), c.lane, c.rab(), c.rcf(),
+                       [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
 }
 
 template <class M>
 __device__ void wave_solve(const WaveCell<M> &c) {
   double *sh = c.sh;
-  run_prog<1, false, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_SF), c.lane, nullptr, nullptr,
-                                     [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
-  run_prog<1, true, 8 * M::WV_ZERO>((char *)sh, M::wprog(WP_SB), c.lane, nullptr, nullptr,
-                                    [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
+  run_prog<1, false, 1, 0u>((char *)sh, M::wprog(WP_SF), c.lane, c.rab(), c.rcf(),
+                        [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
+  run_prog<1, true, 1, 0u>((char *)sh, M::wprog(WP_SB), c.lane, c.rab(), c.rcf(),
+                       [&](unsigned d, unsigned, double v) { ld_st(sh, d, v); });
 }
 
-// The cell's constant slots: +0.0 in LDS and in the scratch's pad slot.
+// The cell's constant slots: +0.0 and +1.0 in LDS, +0.0 in the scratch's
+// pad slot.
 template <class M>
 __device__ __forceinline__ void wave_zero_slots(const WaveCell<M> &c) {
   if (c.lane == 0) {
     c.sh[M::WV_ZERO] = 0.0;
+    c.sh[M::WV_ONE] = 1.0;
     c.ab()[M::WV_NAB] = 0.0;
   }
 }

@@ -230,8 +230,8 @@ def emit(name):
     # cell-per-wave solver: LDS layout, global scratch layout and programs
     prog = wave_sched.build_wave_program(m, t)
     L = prog["layout"]
-    out.append("constexpr int WV_ZERO = %d, WV_X0 = %d, WV_Y0 = %d, WV_SH = %d;"
-               % (L["ZERO"], L["X0"], L["Y0"], L["SH_SIZE"]))
+    out.append("constexpr int WV_ZERO = %d, WV_ONE = %d, WV_X0 = %d, WV_Y0 = %d, WV_SINK = %d, "
+               "WV_SH = %d;" % (L["ZERO"], L["ONE"], L["X0"], L["Y0"], L["SINK"], L["SH_SIZE"]))
     out.append("constexpr int WV_G_AB = %d, WV_NAB = %d, WV_G_F0 = %d, WV_G_K = %d, WV_G_SIZE = %d, "
                "WV_NCOEF = %d;" % (L["G_AB"], L["NAB"], L["G_F0"], L["G_K"], L["G_SIZE"],
                                    len(prog["coef"])))
@@ -242,15 +242,15 @@ def emit(name):
         out.append("constexpr int WV_%s_NB = %d, WV_%s_NR = %d;"
                    % (P.upper(), cmap.get("WV_%s_NB" % P.upper(), 0), P.upper(),
                       cmap.get("WV_%s_NR" % P.upper(), 0)))
-    for aname, ctype, vals in arrays:   # term words are read as 16-byte blocks
+    for aname, ctype, vals in arrays:   # program words are read as 16-byte blocks
         out.append(arr(ctype, aname, vals, per_line=6 if ctype == "double" else 16,
                        fmt=dfmt if ctype == "double" else str,
-                       align=16 if aname.endswith("_term") else 0))
+                       align=16 if aname.endswith("_ops") else 0))
     for P in ("fv", "jv", "lu", "sf", "sb"):   # empty programs of a mechanism without an LU
         if P not in have:
-            for suf in ("term", "flag", "head"):
+            for suf in ("ops", "flag"):
                 out.append(arr("unsigned", "wv_%s_%s" % (P, suf), [0, 0, 0, 0],
-                               align=16 if suf == "term" else 0))
+                               align=16 if suf == "ops" else 0))
     # persistent lane solver: operand-cache schedules of the LU and
     # substitution sections (lane_cache.py), 16 pad words at the end so a
     # record's words can be loaded whole

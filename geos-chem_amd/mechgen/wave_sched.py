@@ -22,20 +22,29 @@ coefficient table (element 0 is -0.0); a step is acc = fma(b, a, acc) from
 acc = +0.0.  lu, sf, sb: both operands are LDS elements; a step is
 acc = fma(-a, b, acc) from acc = the destination slot.
 
-Encoding (the solver reads BYTE offsets, element index * 8; the LDS block
-is below 64 KB, so every offset fits 16 bits):
-  wv_<P>_term  [NB + pad][64][BSTEP] words, block-major then lane: a | b << 16;
-               a pad step reads (ZERO, ZERO) in LDS phases and (NAB, 0) in
-               fv/jv -- products -0.0 that leave every acc unchanged.  The
-               array ends with PF whole pad blocks (the solver loads PF
-               blocks ahead) and NB is padded to a multiple of PF.
-  wv_<P>_flag  [(NB + pad)/8] words: 4 bits per block (block k in bits
-               4(k%8)..4(k%8)+3): 1 first block of a round, 2 last block of a
-               round, 4 the round ends a level (an ordering point follows).
-  wv_<P>_head  [(NR + HPAD)][64] words per round and lane: destination byte
-               offset | divisor byte offset << 16 (0xFFFF: none; jv: 1 for
-               a diagonal entry); destination 0xFFFF: lane idle; HPAD idle
-               rounds end the array (the solver loads heads ahead).
+Encoding (the solver reads BYTE offsets, element index * 8):
+  wv_<P>_ops   [NB + PF + 4][2][64] uint4 words: block k, half h, lane: the
+               lane's ops 2h and 2h+1 of the block, (a, b, a', b').  A
+               round is its header op followed by its terms, padded with
+               pad ops to whole blocks of BSTEP ops.  Header op: LU and
+               substitutions (dest, divisor) -- the partial sum starts at
+               LDS[dest], the result is divided by LDS[divisor], the +1.0
+               slot when the lane does not divide; Vdot and Jacobian
+               (dest, diagonal flag) with the sum starting at +0.0.  An idle
+               lane's header names its own sink slot (SINK + lane).  Term
+               op (a, b): LDS byte offsets (LU, substitutions), or the
+               reaction-scratch and coefficient byte offsets (Vdot,
+               Jacobian).  Pad op: (ZERO, ZERO) in LDS phases and (NAB, 0)
+               in Vdot/Jacobian -- products -0.0 that leave every sum
+               unchanged.  NB is padded to a multiple of PF (the first
+               padding block starts a round of sink headers: the solver
+               stores a lane's value after every block); PF + 4 pad blocks
+               follow (the solver loads program words PF blocks and
+               operands up to 3 blocks ahead).
+  wv_<P>_flag  4 bits per block (block k in bits 4(k%8)..4(k%8)+3 of word
+               k/8): 1 first block of a round, 2 last block of a round,
+               4 the round ends a level (an ordering point follows), 8 some
+               lane of the round divides (LU, backward substitution).
   WV_<P>_NB    blocks (a multiple of PF), WV_<P>_NR rounds.
 Phases P: fv (Vdot), jv (Jacobian in LU layout), lu (row-Crout), sf
 (forward substitution), sb (backward substitution).
@@ -46,24 +55,26 @@ LANES = 64
 IDLE = 0xFFFF
 
 
-BSTEP = 4    # steps per block (one 16-byte load of term words per lane)
-PF = 8       # blocks the solver loads ahead; NB is a multiple of it
-HPAD = 4     # idle rounds after the last (the solver loads heads ahead)
+BSTEP = 4    # ops per block (two 16-byte loads of program words per lane)
+PF = 8       # blocks of program words the solver loads ahead; NB is a multiple of it
+XPAD = 4     # pad blocks past NB + PF (operands are loaded up to 3 blocks ahead)
+F_START, F_END, F_LEVEL, F_DIV = 1, 2, 4, 8
 
 
 def layout(m, t):
     """Element offsets of the cell-per-wave solver's LDS block (LU, a +0.0
-    slot, the solve vector X -- also the stage point Ynew and the scaled
-    error terms --, the state Y) and of a wave's global scratch (rate
-    constants, reaction products / partials, a +0.0 slot, Fcn0, the four
-    stage vectors K)."""
+    slot, a +1.0 slot, the solve vector X -- also the stage point Ynew and
+    the scaled error terms --, the state Y, one sink slot per lane) and of
+    a wave's global scratch (rate constants, reaction products / partials,
+    a +0.0 slot, Fcn0, the four stage vectors K)."""
     nvar, nspec, nreact = m["nvar"], m["nspec"], m["nreact"]
     lu_nz = m["lu_nonzero"] if m["lu_irow"] else 1
     nab = max(nreact, len(t["pt_r"]))
-    L = {"LU0": 0, "ZERO": lu_nz}
-    L["X0"] = lu_nz + 1
+    L = {"LU0": 0, "ZERO": lu_nz, "ONE": lu_nz + 1}
+    L["X0"] = lu_nz + 2
     L["Y0"] = L["X0"] + nvar
-    L["SH_SIZE"] = L["Y0"] + nspec
+    L["SINK"] = L["Y0"] + nspec
+    L["SH_SIZE"] = L["SINK"] + LANES
     assert 8 * L["SH_SIZE"] < IDLE
     L["G_RCT"] = 0
     L["G_AB"] = nreact
@@ -201,54 +212,77 @@ def build_wave_program(m, t):
 
 
 def encode(prog, P):
-    """The solver's arrays of phase P (see the module docstring): term words
-    [NB + PF][64][BSTEP], flag words, head words, NB, NR."""
+    """The solver's arrays of phase P (see the module docstring): ops words
+    [NB + PF + XPAD][2][64][4], flag words, NB, NR."""
     (lev, rs, head, term_rows), pad = prog[P]
+    L = prog["layout"]
     nr = len(rs) - 1
     lds = P not in ("fv", "jv")
-    blocks, flags = [], []
     lev_end = set(lev[1:])
+    blocks, flags = [], []
+    padop = (8 * L["ZERO"], 8 * L["ZERO"]) if lds else (8 * pad[0], 8 * pad[1])
+
+    def hdr(r, ln):
+        h = head[r * LANES + ln]
+        d, hi = h & 0xFFFF, h >> 16
+        if d == IDLE:
+            return (8 * (L["SINK"] + ln), 8 * L["ONE"] if lds else 0)
+        if not lds:
+            return (8 * d, hi)
+        if P in ("lu", "sb") and hi:
+            return (8 * d, 8 * (hi - 1))
+        return (8 * d, 8 * L["ONE"])
+
     for r in range(nr):
         rows = term_rows[rs[r]:rs[r + 1]]
-        nblk = max(1, -(-len(rows) // BSTEP))
-        rows = rows + [[None] * LANES] * (nblk * BSTEP - len(rows))
-        for b in range(nblk):
-            blocks.append(rows[b * BSTEP:(b + 1) * BSTEP])
-            f = (1 if b == 0 else 0) | (2 if b == nblk - 1 else 0)
-            if b == nblk - 1 and (r + 1) in lev_end:
-                f |= 4
+        nops = 1 + len(rows)
+        nblk = -(-nops // BSTEP)
+        lanes = []
+        divides = False
+        for ln in range(LANES):
+            ops = [hdr(r, ln)]
+            for row in rows:
+                ab = row[ln]
+                ops.append((8 * ab[0], 8 * ab[1]) if ab is not None else padop)
+            ops += [padop] * (nblk * BSTEP - len(ops))
+            lanes.append(ops)
+            h = head[r * LANES + ln]
+            if P in ("lu", "sb") and (h & 0xFFFF) != IDLE and (h >> 16):
+                divides = True
+        for bk in range(nblk):
+            blocks.append([lanes[ln][bk * BSTEP:(bk + 1) * BSTEP] for ln in range(LANES)])
+            f = (F_START if bk == 0 else 0) | (F_END if bk == nblk - 1 else 0)
+            if bk == nblk - 1 and (r + 1) in lev_end:
+                f |= F_LEVEL
+            if bk == nblk - 1 and divides:
+                f |= F_DIV
             flags.append(f)
     nb = len(blocks)
     nb_pad = -(-nb // PF) * PF
-    empty = [[None] * LANES] * BSTEP
-    blocks += [empty] * (nb_pad - nb + PF)
-    flags += [0] * (nb_pad - nb + PF)
-
-    def word(ab):
-        a, b = ab if ab is not None else pad
-        return (8 * int(a)) | ((8 * int(b)) << 16)
-    term = []
+    empty = [[padop] * BSTEP for _ in range(LANES)]
+    blocks += [empty] * (nb_pad - nb + PF + XPAD)
+    flags += [0] * (nb_pad - nb + PF + XPAD)
+    if nb_pad > nb:
+        # the solver stores a lane's value after every block: the blocks
+        # that pad the last round's end to PF start a round of sink
+        # headers, so they store into the sinks
+        blocks[nb] = [[(8 * (L["SINK"] + ln), 8 * L["ONE"] if lds else 0)] + [padop] * (BSTEP - 1)
+                      for ln in range(LANES)]
+        flags[nb] = F_START
+    flags += [0] * (-len(flags) % 8)
+    ops = []
     for blk in blocks:
-        for ln in range(LANES):
-            for st in range(BSTEP):
-                term.append(word(blk[st][ln]))
+        for h in range(2):
+            for ln in range(LANES):
+                o0, o1 = blk[ln][2 * h], blk[ln][2 * h + 1]
+                ops += [o0[0], o0[1], o1[0], o1[1]]
     flag = []
     for k in range(0, len(flags), 8):
         w = 0
         for i, f in enumerate(flags[k:k + 8]):
             w |= f << (4 * i)
         flag.append(w)
-
-    def hword(h):
-        h = int(h)
-        d, hi = h & 0xFFFF, h >> 16
-        if d == IDLE:
-            return IDLE
-        if P in ("lu", "sb"):
-            hi = 8 * (hi - 1) if hi else IDLE
-        return (8 * d) | (hi << 16)
-    heads = [hword(h) for h in head] + [IDLE] * (HPAD * LANES)
-    return term, flag, heads, nb_pad, nr
+    return ops, flag, nb_pad, nr
 
 
 def emit_arrays(prog):
@@ -258,10 +292,9 @@ def emit_arrays(prog):
     for P in ("fv", "jv", "lu", "sf", "sb"):
         if P not in prog:
             continue
-        term, flag, heads, nb, nr = encode(prog, P)
-        arrays.append(("wv_%s_term" % P, "unsigned", term))
+        ops, flag, nb, nr = encode(prog, P)
+        arrays.append(("wv_%s_ops" % P, "unsigned", ops))
         arrays.append(("wv_%s_flag" % P, "unsigned", flag))
-        arrays.append(("wv_%s_head" % P, "unsigned", heads))
         consts.append(("WV_%s_NB" % P.upper(), nb))
         consts.append(("WV_%s_NR" % P.upper(), nr))
     arrays.append(("wv_coef", "double", prog["coef"]))
@@ -273,38 +306,74 @@ def emit_arrays(prog):
 # arithmetic for one step, to show it performs the same operations in the
 # same order as the sequential table walk.
 def run_phase(prog, P, sh, step, init_from_dest, store, ab=None, coef=None):
-    """Executes phase P from its encoded arrays as the solver does: block by
-    block, every lane's four steps, a round's result stored at its last
+    """Executes phase P from its encoded arrays in the solver's order
+    (csrc/wave_solver.hip, run_prog): block by block, the operands of a
+    block read D blocks before it runs (D = 3 for Vdot and Jacobian, 1
+    otherwise; in LU and substitutions the reads for the block after a
+    level's last block wait until that block has run), a round's header op
+    naming the start of its sum, and every lane's value stored after every
     block.  sh: the LDS block (element list, modified in place); ab, coef:
     the reaction scratch (with its +0.0 pad slot) and the coefficient table
     (fv, jv operands).  step(acc, a, b) -> acc; init_from_dest: acc starts
-    at sh[dest] (else +0.0); store(sh, dest, hi, acc) writes the result
-    (element indices; hi: divisor element + 1, or the Jacobian's diagonal
-    flag, 0 for none)."""
-    term, flag, heads, nb, nr = encode(prog, P)
+    at sh[dest] (else +0.0); store(sh, dest, hi, acc) writes a value
+    (element indices; hi: divisor element + 1 when the round's last block
+    divides and the divisor is not the +1.0 slot, the Jacobian's diagonal
+    flag, else 0)."""
+    ops, flag, nb, nr = encode(prog, P)
+    L = prog["layout"]
     lds = P not in ("fv", "jv")
-    r = 0
-    acc = [0.0] * LANES
-    for k in range(nb):
-        f = (flag[k // 8] >> (4 * (k % 8))) & 15
+    D = 1 if lds else 3
+    NAB = L["NAB"]
+
+    def flag_of(k):
+        return (flag[k // 8] >> (4 * (k % 8))) & 15
+
+    def fetch(k):
+        f = flag_of(k)
+        out = []
         for ln in range(LANES):
-            h = heads[r * LANES + ln]
-            dest = h & 0xFFFF
-            if f & 1:
-                acc[ln] = sh[dest // 8] if (init_from_dest and dest != IDLE) else 0.0
-            for st in range(BSTEP):
-                w = term[(k * LANES + ln) * BSTEP + st]
-                a, b = w & 0xFFFF, w >> 16
+            w = [ops[((2 * k + h) * LANES + ln) * 4 + j] for h in range(2) for j in range(4)]
+            u = [(w[0], w[1]), (w[2], w[3]), (w[4], w[5]), (w[6], w[7])]
+            vals = []
+            for q, (a, b) in enumerate(u):
                 if lds:
-                    acc[ln] = step(acc[ln], sh[a // 8], sh[b // 8])
+                    vals.append((sh[a // 8], sh[b // 8]))
+                elif q == 0 and f & F_START:
+                    vals.append((ab[NAB], coef[0]))
                 else:
-                    acc[ln] = step(acc[ln], ab[a // 8], coef[b // 8])
-            if f & 2 and dest != IDLE:
-                hi = h >> 16
-                if P in ("lu", "sb"):
-                    hi = 0 if hi == IDLE else hi // 8 + 1
-                store(sh, dest // 8, hi, acc[ln])
-        if f & 2:
+                    vals.append((ab[a // 8], coef[b // 8]))
+            out.append((u[0], vals))
+        return out
+
+    pending = {k: fetch(k) for k in range(D)}
+    acc = [0.0] * LANES
+    cur = [(0, 0)] * LANES
+    dv = [1.0] * LANES
+    r = 0
+    for k in range(nb):
+        f = flag_of(k)
+        hold = lds and (f & F_LEVEL)
+        if not hold:
+            pending[k + D] = fetch(k + D)
+        blk = pending.pop(k)
+        for ln in range(LANES):
+            hdr, vals = blk[ln]
+            if f & F_START:
+                cur[ln] = hdr
+                acc[ln] = vals[0][0] if init_from_dest else 0.0
+                dv[ln] = vals[0][1]
+                vals = vals[1:]
+            for a, b in vals:
+                acc[ln] = step(acc[ln], a, b)
+            d, x = cur[ln]
+            if lds:
+                div = (f & F_END) and (f & F_DIV) and x != 8 * L["ONE"]
+                store(sh, d // 8, x // 8 + 1 if div else 0, acc[ln])
+            else:
+                store(sh, d // 8, x, acc[ln])
+        if hold:
+            pending[k + D] = fetch(k + D)
+        if f & F_END:
             r += 1
     assert r == nr
     return sh

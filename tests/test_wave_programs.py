@@ -34,6 +34,7 @@ def fresh_lds(prog, rng):
     L = prog["layout"]
     sh = list(rng.uniform(-1.0, 1.0, L["SH_SIZE"]))
     sh[L["ZERO"]] = 0.0
+    sh[L["ONE"]] = 1.0
     ab = list(rng.uniform(-1.0, 1.0, L["NAB"] + 1))
     ab[L["NAB"]] = 0.0
     return sh, L, ab, list(prog["coef"])
@@ -154,36 +155,53 @@ def test_emitted_tables_match_programs(fullchem):
 
 @pytest.mark.parametrize("name", ["fullchem", "Hg", "carbon"])
 def test_program_offsets_inside_their_blocks(name, mech_of):
-    """Every byte offset an encoded program names -- destination, divisor,
-    both operands of every step, pads included -- lies inside its block: the
-    LDS block for LU and substitution operands and every destination, the
-    reaction scratch (with its pad slot) and the coefficient table for the
-    Vdot and Jacobian operands; destinations never touch the +0.0 slot or
-    the state Y; the block flags mark every round's first and last block;
-    the term array holds PF pad blocks past NB, NB a multiple of PF."""
+    """Every byte offset an encoded program names -- header destination and
+    divisor, both operands of every op, pads included -- lies inside its
+    block: the LDS block for LU and substitution operands and every
+    destination, the reaction scratch (with its pad slot) and the
+    coefficient table for the Vdot and Jacobian operands; destinations
+    never touch the +0.0 or +1.0 slots or the state Y, and an idle lane's
+    destination is its own sink slot; the block flags mark every round's
+    first and last block; the ops array holds PF + XPAD pad blocks past NB,
+    NB a multiple of PF, and the flag words cover them."""
     W, t, prog = programs(mech_of(name))
     L = prog["layout"]
     sh8, ab8, cf8 = 8 * L["SH_SIZE"], 8 * (L["NAB"] + 1), 8 * len(prog["coef"])
+    sink = 8 * L["SINK"]
     for P in ("fv", "jv", "lu", "sf", "sb"):
         if P not in prog:
             continue
-        term, flag, heads, nb, nr = W.encode(prog, P)
-        assert nb % W.PF == 0 and len(term) == (nb + W.PF) * 64 * W.BSTEP
-        assert len(heads) == (nr + W.HPAD) * 64 and len(flag) * 8 >= nb + W.PF
-        fl = [(flag[k // 8] >> (4 * (k % 8))) & 15 for k in range(nb + W.PF)]
-        assert sum(f & 1 for f in fl) == nr and sum((f >> 1) & 1 for f in fl) == nr
+        ops, flag, nb, nr = W.encode(prog, P)
+        nblk = nb + W.PF + W.XPAD
+        assert nb % W.PF == 0 and len(ops) == nblk * 2 * 64 * 4
+        assert len(flag) * 8 >= nblk
+        fl = [(flag[k // 8] >> (4 * (k % 8))) & 15 for k in range(len(flag) * 8)]
+        real = max(k for k in range(nblk) if fl[k] & 2) + 1     # blocks of the rounds
+        assert sum(f & 1 for f in fl[:real]) == nr and sum((f >> 1) & 1 for f in fl) == nr
+        # the padding to NB starts a round of sink headers (no end); past NB nothing
+        assert fl[real:nb] == ([1] + [0] * (nb - real - 1) if nb > real else [])
         assert all(f == 0 for f in fl[nb:])
-        for h in heads:
-            d, hi = h & 0xFFFF, h >> 16
-            if d == W.IDLE:
-                continue
-            assert d % 8 == 0 and (d < 8 * L["ZERO"] or 8 * L["X0"] <= d < 8 * L["Y0"])
-            if P in ("lu", "sb") and hi != W.IDLE:
-                assert hi % 8 == 0 and hi < 8 * L["ZERO"]
-        for w in term:
-            a, b = w & 0xFFFF, w >> 16
-            assert a % 8 == 0 and b % 8 == 0
-            if P in ("fv", "jv"):
-                assert a < ab8 and b < cf8
-            else:
-                assert a < sh8 and b < sh8
+        lds = P not in ("fv", "jv")
+        start = False
+        for k in range(nblk):
+            f = fl[k]
+            for ln in range(64):
+                w = [ops[((2 * k + h) * 64 + ln) * 4 + j] for h in range(2) for j in range(4)]
+                u = [(w[0], w[1]), (w[2], w[3]), (w[4], w[5]), (w[6], w[7])]
+                if f & 1:
+                    d, x = u[0]
+                    assert d % 8 == 0
+                    if k >= real:
+                        assert d == sink + 8 * ln
+                    assert (d < 8 * L["ZERO"] or 8 * L["X0"] <= d < 8 * L["Y0"] or d == sink + 8 * ln)
+                    if lds:
+                        assert x % 8 == 0 and (x < 8 * L["ZERO"] or x == 8 * L["ONE"])
+                    else:
+                        assert x in (0, 1)
+                    u = u[1:]
+                for a, b in u:
+                    assert a % 8 == 0 and b % 8 == 0
+                    if lds:
+                        assert a < sh8 and b < sh8 and a < sink and b < sink
+                    else:
+                        assert a < ab8 and b < cf8
