@@ -1,6 +1,6 @@
 """A KPP mechanism as this project describes it (mech/<name>.json, written by
-mechgen/kpp_parse.py from the reference's KPP tables, plus
-mech/<name>_species.json from mechgen/species_db.py)."""
+the mechanism compiler mechgen/eqn_compile.py from the mechanism's KPP
+source, plus mech/<name>_species.json from mechgen/species_db.py)."""
 import json
 import re
 from functools import lru_cache
@@ -13,15 +13,21 @@ MECH_IDS = {"fullchem": 0, "Hg": 1, "carbon": 2}
 
 
 class Mechanism:
-    def __init__(self, name):
-        d = json.loads((_MECH_DIR / ("%s.json" % name)).read_text())
+    def __init__(self, name, d=None, species_consts=None):
+        """The committed mechanism `name`, or the description d (the JSON
+        schema of mechgen/kpp_parse.py and mechgen/eqn_compile.py) with its
+        species constants (same schema as mech/<name>_species.json)."""
+        if d is None:
+            d = json.loads((_MECH_DIR / ("%s.json" % name)).read_text())
         self.name = name
         self.d = d
         self.nspec, self.nvar, self.nfix = d["nspec"], d["nvar"], d["nfix"]
         self.nreact, self.lu_nonzero = d["nreact"], d["lu_nonzero"]
         self.species = d["species"]
         self.reactions = d["reactions"]
-        self.species_consts = json.loads((_MECH_DIR / ("%s_species.json" % name)).read_text())
+        if species_consts is None:
+            species_consts = json.loads((_MECH_DIR / ("%s_species.json" % name)).read_text())
+        self.species_consts = species_consts
         self.lu_irow = np.asarray(d["lu_irow"], dtype=np.int32)
         self.lu_icol = np.asarray(d["lu_icol"], dtype=np.int32)
         self.lu_crow = np.asarray(d["lu_crow"], dtype=np.int32)
@@ -61,3 +67,14 @@ class Mechanism:
 @lru_cache(maxsize=None)
 def load(name):
     return Mechanism(name)
+
+
+def reordered(mech, d):
+    """The mechanism of description d, whose species are mech's in another
+    order (a compiler's own ordering): species constants permuted by name."""
+    pos = {n: i for i, n in enumerate(mech.species)}
+    perm = [pos[n] for n in d["species"]]
+    sc = {}
+    for k, v in mech.species_consts.items():
+        sc[k] = [v[i] for i in perm] if isinstance(v, list) and len(v) == mech.nspec else v
+    return Mechanism(mech.name, d, sc)
