@@ -14,6 +14,7 @@
 #include "../../include/gckpp_hip.h"
 #include "cell_fields.h"
 #include "gckpp_internal.h"
+#include "gen/species_fullchem.h"
 #include "mlp.h"
 
 namespace gckpp {
@@ -24,6 +25,9 @@ bool rates_available(int mech);
 using namespace gckpp;
 
 static thread_local char g_err[512];
+
+// Avogadro's number, Headers/physconstants.F90:32
+constexpr double kAvogadro = 6.022140857e+23;
 
 // Records the message of a failed call.  The HIP runtime's last-error state
 // is consumed too: a failure reported here (a failed allocation, say) must
@@ -92,6 +96,12 @@ struct gckpp_ctx {
   int64_t hwaves, hw_fields;
   bool ho_timed;         // ev_ho recorded by a launch
   int64_t na_waves;      // resident waves of the cell-per-wave kernel (no-allocation entry)
+  // gckpp_set_species_map: the model species of each KPP species (-1:
+  // none), the unit-conversion factors, the alkalinity-equivalent factors
+  // (1 for the other species), the prod/loss dummies (device)
+  int32_t *sm_map, *sm_zero;
+  double *sm_cin, *sm_cout, *sm_eq;
+  int32_t sm_nmodel;
 };
 
 extern "C" const char *gckpp_last_error(void) { return g_err; }
@@ -251,7 +261,8 @@ extern "C" int gckpp_finalize(gckpp_ctx *c) {
   for (void *p : {(void *)c->work, (void *)c->atol, (void *)c->rtol, (void *)c->consts,
                   (void *)c->step_buf, (void *)c->nfail, (void *)c->counter, (void *)c->hs_d,
                   (void *)c->hs_i, (void *)c->wwork, (void *)c->wcounter, (void *)c->ho_ctr,
-                  (void *)c->ho_cell, (void *)c->ho_rec, (void *)c->hwork})
+                  (void *)c->ho_cell, (void *)c->ho_rec, (void *)c->hwork, (void *)c->sm_map,
+                  (void *)c->sm_zero, (void *)c->sm_cin, (void *)c->sm_cout, (void *)c->sm_eq})
     if (p) ok = hipFree(p) == hipSuccess && ok;
   if (c->s2) ok = hipStreamDestroy(c->s2) == hipSuccess && ok;
   if (c->ev_fork) ok = hipEventDestroy(c->ev_fork) == hipSuccess && ok;
@@ -891,6 +902,112 @@ extern "C" int gckpp_rates(gckpp_ctx *c, int64_t ncell, const double *met, const
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(stage_out(rconst, v.rconst, c->info.nreact, n, ncell, c0));
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The model side of the step: Map_KppSpc (fullchem_mod.F90:817-821), the
+// unit conversions of Convert_Spc_Units (GeosUtil/unitconv_mod.F90:1287-1391,
+// 1407-1515) and the prod/loss dummies zeroed before the rates (:942-946).
+extern "C" int gckpp_set_species_map(gckpp_ctx *c, int32_t nmodel, const int32_t *map,
+                                     const double *mw_g, const int32_t *zero, int32_t alk_equiv) {
+  if (!c || !map || !mw_g) return fail("gckpp_set_species_map: null argument");
+  if (nmodel <= 0) return fail("gckpp_set_species_map: nmodel must be positive");
+  const int ns = c->info.nspec;
+  std::vector<double> cin(ns, 0.0), cout(ns, 0.0), eq(ns, 1.0);
+  if (alk_equiv && c->mech != MECH_FULLCHEM)
+    return fail("gckpp_set_species_map: the alkalinity conversion is fullchem's (mechanism %d)",
+                c->mech);
+  std::vector<int32_t> zz(ns, 0);
+  for (int k = 0; k < ns; ++k) {
+    const int s = map[k];
+    if (s < -1 || s >= nmodel)
+      return fail("gckpp_set_species_map: KPP species %d maps to model species %d of %d", k, s, nmodel);
+    if (s < 0) continue;
+    if (!(mw_g[s] > 0.0))
+      return fail("gckpp_set_species_map: model species %d has molecular weight %g", s, mw_g[s]);
+    const double mw_kg = mw_g[s] * 1.0e-3;
+    cin[k] = (kAvogadro / mw_kg) / 1.0e+6;
+    cout[k] = 1.0e+6 / (kAvogadro / mw_kg);
+    zz[k] = zero ? (zero[k] != 0) : 0;
+    // fullchem_ConvertAlkToEquiv (KPP/fullchem/fullchem_SulfurChemFuncs.F90:
+    // 67-78): C * (MW * 7e-5) for the sea-salt alkalinity tracers; KPP's MW
+    // is the model species' molecular weight
+    if (alk_equiv && (k == species_fullchem::SALAAL || k == species_fullchem::SALCAL))
+      eq[k] = mw_g[s] * 7.0e-5;
+  }
+  if (!c->sm_map) {
+    HIP_OK(hipMalloc(&c->sm_map, sizeof(int32_t) * ns));
+    HIP_OK(hipMalloc(&c->sm_zero, sizeof(int32_t) * ns));
+    HIP_OK(hipMalloc(&c->sm_cin, sizeof(double) * ns));
+    HIP_OK(hipMalloc(&c->sm_cout, sizeof(double) * ns));
+    HIP_OK(hipMalloc(&c->sm_eq, sizeof(double) * ns));
+  }
+  HIP_OK(hipMemcpy(c->sm_map, map, sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->sm_zero, zz.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->sm_cin, cin.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->sm_cout, cout.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->sm_eq, eq.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+  c->sm_nmodel = nmodel;
+  return 0;
+}
+
+extern "C" int gckpp_gather_dev(gckpp_ctx *c, int64_t ncell, const double *species,
+                                const double *airden, double *conc, void *stream) {
+  if (!c) return fail("gckpp_gather_dev: null context");
+  if (ncell < 0) return fail("gckpp_gather_dev: negative ncell");
+  if (ncell == 0) return 0;
+  if (!c->sm_map) return fail("gckpp_gather_dev: no species map (gckpp_set_species_map)");
+  if (!species || !airden || !conc) return fail("gckpp_gather_dev: null argument");
+  launch_gather(ncell, c->info.nspec, c->sm_map, c->sm_cin, c->sm_eq, c->sm_zero, species, airden,
+                conc,
+                (hipStream_t)stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gckpp_scatter_dev(gckpp_ctx *c, int64_t ncell, const double *conc,
+                                 const double *airden, double *species, void *stream) {
+  if (!c) return fail("gckpp_scatter_dev: null context");
+  if (ncell < 0) return fail("gckpp_scatter_dev: negative ncell");
+  if (ncell == 0) return 0;
+  if (!c->sm_map) return fail("gckpp_scatter_dev: no species map (gckpp_set_species_map)");
+  if (!species || !airden || !conc) return fail("gckpp_scatter_dev: null argument");
+  launch_scatter(ncell, c->info.nspec, c->sm_map, c->sm_cout, c->sm_eq, conc, airden, species,
+                 (hipStream_t)stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gckpp_set_state_het_dev(gckpp_ctx *c, int64_t ncell, const double *hetin,
+                                       const double *conc, int32_t naerotype, double gamma_ho2,
+                                       int32_t lpscchem, int32_t turn_off_het, double *het,
+                                       void *stream) {
+  if (!c) return fail("gckpp_set_state_het_dev: null context");
+  if (c->mech != MECH_FULLCHEM)
+    return fail("gckpp_set_state_het_dev: the HetState record is fullchem's (context mechanism %d)",
+                c->mech);
+  if (ncell < 0) return fail("gckpp_set_state_het_dev: negative ncell");
+  if (naerotype < 0 || naerotype > 14)
+    return fail("gckpp_set_state_het_dev: nAeroType %d outside 0..14", naerotype);
+  if (ncell == 0) return 0;
+  if (!hetin || !conc || !het) return fail("gckpp_set_state_het_dev: null argument");
+  launch_set_state_het(ncell, hetin, conc, het, naerotype, gamma_ho2, lpscchem, turn_off_het,
+                       (hipStream_t)stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gckpp_sulfur_aq_dev(gckpp_ctx *c, int64_t ncell, const double *hetin,
+                                   const double *conc, double *met, void *stream) {
+  if (!c) return fail("gckpp_sulfur_aq_dev: null context");
+  if (c->mech != MECH_FULLCHEM)
+    return fail("gckpp_sulfur_aq_dev: K_MT is fullchem's (context mechanism %d)", c->mech);
+  if (ncell < 0) return fail("gckpp_sulfur_aq_dev: negative ncell");
+  if (ncell == 0) return 0;
+  if (!hetin || !conc || !met) return fail("gckpp_sulfur_aq_dev: null argument");
+  launch_sulfur_aq(ncell, hetin, conc, c->consts, met, (hipStream_t)stream);
+  HIP_OK(hipGetLastError());
   return 0;
 }
 

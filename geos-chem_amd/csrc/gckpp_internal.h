@@ -109,6 +109,22 @@ struct CompArgs {
 };
 
 MechInfo mech_info(int mech);
+// producers.hip: the gather of the KPP vector from the model's species
+// (kg/kg dry) and the scatter back
+// fullchem_SetStateHet per cell (producers.hip): hetin[NHIN][n], conc[NSPEC][n]
+// of fullchem -> het[NHET][n].
+void launch_set_state_het(int64_t n, const double *hetin, const double *conc, double *het,
+                          int naerotype, double gamma_ho2, int lpscchem, int turn_off_het,
+                          hipStream_t s);
+// fullchem_SulfurAqChem per cell (producers.hip): K_MT(1:6) into met[MET_K_MT..]
+// from hetin, conc and met's TEMP and NUMDEN; consts = the context's MW table.
+void launch_sulfur_aq(int64_t n, const double *hetin, const double *conc, const double *consts,
+                      double *met, hipStream_t s);
+void launch_gather(int64_t n, int nspec, const int32_t *map, const double *cin, const double *eq,
+                   const int32_t *zero, const double *species, const double *airden, double *conc,
+                   hipStream_t s);
+void launch_scatter(int64_t n, int nspec, const int32_t *map, const double *cout, const double *eq,
+                    const double *conc, const double *airden, double *species, hipStream_t s);
 int set_error(const char *msg);  // gckpp_last_error's message; returns -1
 // doubles of per-cell workspace the integrator needs
 inline int64_t integrate_work_per_cell(const MechInfo &m) {

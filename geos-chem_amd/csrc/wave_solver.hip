@@ -137,11 +137,22 @@ __device__ __forceinline__ void run_prog(char *shb, const WProg &p, int lane,
     acc = term(oa[sl][3], ob[sl][3]);
     double v = acc;
     if constexpr (DIV) {
-      if ((f & (F_END | F_DIV)) == (F_END | F_DIV)) v = v / dv;
+      if ((f & (F_END | F_DIV)) == (F_END | F_DIV)) {
+        // (the empty asm keeps the division inside the branch: computed
+        // for every block and selected, it costs ~14 VALU per block)
+        asm volatile("" : "+v"(v));
+        v = v / dv;
+      }
     }
     store(dest, xw, v);
   };
   unsigned fl = p.flag[0];
+  // The flags of the next kPF blocks come through a vector (buffer) load:
+  // a scalar load in the loop would be outstanding while the LDS operands
+  // are awaited, and scalar loads complete out of order, so every wait for
+  // an LDS operand would have to drain the whole LDS queue (lgkmcnt(0)),
+  // including the operands just requested for the next block.
+  const __amdgpu_buffer_rsrc_t rfl = brsrc(p.flag, 4 * (p.nb / kPF + 1));
   // blocks 0..D-1; the ring then holds blocks D..kPF+D-1
 #pragma unroll
   for (int j = 0; j < D; ++j) {
@@ -151,25 +162,32 @@ __device__ __forceinline__ void run_prog(char *shb, const WProg &p, int lane,
   }
 #pragma unroll 1
   for (int k0 = 0; k0 < p.nb; k0 += kPF) {
-    const unsigned fln = p.flag[(k0 >> 3) + 1];
+    // (a VGPR until the group's end, so that the load is awaited late)
+    const unsigned fln = __builtin_amdgcn_raw_buffer_load_b32(rfl, 4 * ((k0 / kPF) + 1), 0, 0);
     const uint4 *tq = tp + (2 * (k0 + kPF)) * 64;  // program words of block k0 + kPF
 #pragma unroll
     for (int i = 0; i < kPF; ++i) {
       const unsigned f = (fl >> (4 * i)) & 15u;
       const int jn = i + D;  // the block whose operands are requested now, relative to k0
       const unsigned fj = jn < kPF ? (fl >> (4 * jn)) & 15u : (fln >> (4 * (jn - kPF))) & 15u;
-      // MODE 1 (D = 1): the next block's operands wait for a level's end
+      // MODE 1 (D = 1): the next block's operands wait for a level's end.
+      // The block runs in both branches, so that each has its own order of
+      // LDS operations and the wait for this block's operands in the usual
+      // branch need not drain the next block's (a wait after a join must
+      // hold for both orders).
       const bool hold = MODE == 1 && (f & F_LEVEL);
-      if (!hold) fetch(jn % R, rw[jn % kPF][0], rw[jn % kPF][1], fj);
-      run(i % R, f);
-      if (hold) {
+      if (!hold) {
+        fetch(jn % R, rw[jn % kPF][0], rw[jn % kPF][1], fj);
+        run(i % R, f);
+      } else {
+        run(i % R, f);
         wave_sync();
         fetch(jn % R, rw[jn % kPF][0], rw[jn % kPF][1], fj);
       }
       rw[jn % kPF][0] = tq[(2 * jn) * 64];
       rw[jn % kPF][1] = tq[(2 * jn + 1) * 64];
     }
-    fl = fln;
+    fl = __builtin_amdgcn_readfirstlane(fln);
   }
   wave_sync();
 }

@@ -1,6 +1,6 @@
 """Per-cell record layouts (met, het) shared with the HIP kernels: parsed from
 the generated C header csrc/cell_fields.h so Python and device code cannot
-disagree.  MET/HET map field name -> (offset, length)."""
+disagree.  MET/HET/HIN map field name -> (offset, length)."""
 import re
 from pathlib import Path
 
@@ -21,4 +21,5 @@ def _parse(prefix, total_name):
 
 MET, NMET = _parse("MET", "NMET")
 HET, NHET = _parse("HET", "NHET")
+HIN, NHIN = _parse("HIN", "NHIN")   # fullchem_SetStateHet's model fields
 NPHOTOL = int(re.search(r"#define NPHOTOL (\d+)", _H.read_text()).group(1))

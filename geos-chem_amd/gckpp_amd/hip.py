@@ -39,6 +39,8 @@ SYMBOLS = [
     "gckpp_persistent_handoff_stats",
     "gckpp_aout_dev", "gckpp_aout", "gckpp_kppsa_write", "gckpp_kppsa_read",
     "gckpp_kpph_restart_write", "gckpp_kpph_restart_read", "gckpp_integrate_noalloc_dev",
+    "gckpp_set_species_map", "gckpp_gather_dev", "gckpp_scatter_dev", "gckpp_set_state_het_dev",
+    "gckpp_sulfur_aq_dev",
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
@@ -112,6 +114,12 @@ def load():
     L.gckpp_solve_dev.argtypes = [vp, i64, dp, dp, vp]
     L.gckpp_aout_dev.argtypes = [vp, i64, dp, dp, dp, vp]
     L.gckpp_aout.argtypes = [vp, i64, dp, dp, dp]
+    L.gckpp_set_species_map.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
+    L.gckpp_gather_dev.argtypes = [vp, i64, dp, dp, dp, vp]
+    L.gckpp_scatter_dev.argtypes = [vp, i64, dp, dp, dp, vp]
+    L.gckpp_sulfur_aq_dev.argtypes = [vp, i64, dp, dp, dp, vp]
+    L.gckpp_set_state_het_dev.argtypes = [vp, i64, dp, dp, ctypes.c_int32, ctypes.c_double,
+                                          ctypes.c_int32, ctypes.c_int32, dp, vp]
     L.gckpp_kppsa_write.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(KppsaMeta),
                                     ctypes.c_char_p, dp, dp, dp, dp]
     L.gckpp_kpph_restart_write.argtypes = [ctypes.c_char_p, i64, i64, i64, dp, ctypes.c_char_p]
@@ -509,6 +517,77 @@ class Context:
         _check(self.L.gckpp_solve_dev(self.h, x.shape[1], _ptr(lu), _ptr(out), _stream()),
                "gckpp_solve_dev")
         return out
+
+    def set_species_map(self, model_mw_g, kpp_to_model, zero=None, alk_equiv=False):
+        """Map_KppSpc: kpp_to_model[NSPEC] (model species index or -1),
+        model_mw_g[nmodel] [g/mol], zero[NSPEC] the prod/loss dummies;
+        alk_equiv: SALAAL/SALCAL to equivalents in the gather and back in
+        the scatter (fullchem)."""
+        mp = np.ascontiguousarray(kpp_to_model, dtype=np.int32)
+        mw = np.ascontiguousarray(model_mw_g, dtype=np.float64)
+        if mp.shape != (self.mech.nspec,):
+            raise GckppError("set_species_map: map of shape %s" % (mp.shape,))
+        zz = None if zero is None else np.ascontiguousarray(zero, dtype=np.int32)
+        _check(self.L.gckpp_set_species_map(self.h, int(mw.size), mp.ctypes.data_as(ctypes.c_void_p),
+                                            mw.ctypes.data_as(ctypes.c_void_p),
+                                            None if zz is None else zz.ctypes.data_as(ctypes.c_void_p),
+                                            int(bool(alk_equiv))),
+               "gckpp_set_species_map")
+
+    def gather(self, species, airden, conc=None):
+        """species[nmodel][ncell] kg/kg dry, airden[ncell] kg/m3 ->
+        conc[NSPEC][ncell] molec/cm3 (device tensors)."""
+        T = self.torch
+        n = species.shape[1]
+        _want(species, "species", (species.shape[0], n), T.float64)
+        _want(airden, "airden", (n,), T.float64)
+        if conc is None:
+            conc = T.empty((self.mech.nspec, n), dtype=T.float64, device=species.device)
+        _want(conc, "conc", (self.mech.nspec, n), T.float64)
+        _check(self.L.gckpp_gather_dev(self.h, n, _ptr(species), _ptr(airden), _ptr(conc), _stream()),
+               "gckpp_gather_dev")
+        return conc
+
+    def scatter(self, conc, airden, species):
+        """The mapped species of conc back into species (kg/kg dry)."""
+        T = self.torch
+        n = conc.shape[1]
+        _want(conc, "conc", (self.mech.nspec, n), T.float64)
+        _want(airden, "airden", (n,), T.float64)
+        _want(species, "species", (species.shape[0], n), T.float64)
+        _check(self.L.gckpp_scatter_dev(self.h, n, _ptr(conc), _ptr(airden), _ptr(species), _stream()),
+               "gckpp_scatter_dev")
+
+    def set_state_het(self, hetin, conc, het=None, naerotype=14, gamma_ho2=0.2, lpscchem=True,
+                      turn_off_het=False):
+        """fullchem_SetStateHet: het[NHET][ncell] from the model fields
+        hetin[NHIN][ncell] and conc[NSPEC][ncell] (device tensors)."""
+        from . import fields
+        T = self.torch
+        n = conc.shape[1]
+        _want(hetin, "hetin", (fields.NHIN, n), T.float64)
+        _want(conc, "conc", (self.mech.nspec, n), T.float64)
+        if het is None:
+            het = T.empty((fields.NHET, n), dtype=T.float64, device=conc.device)
+        _want(het, "het", (fields.NHET, n), T.float64)
+        _check(self.L.gckpp_set_state_het_dev(self.h, n, _ptr(hetin), _ptr(conc), int(naerotype),
+                                              float(gamma_ho2), int(bool(lpscchem)),
+                                              int(bool(turn_off_het)), _ptr(het), _stream()),
+               "gckpp_set_state_het_dev")
+        return het
+
+    def sulfur_aq(self, hetin, conc, met):
+        """fullchem_SulfurAqChem: K_MT(1:6) into met's rows MET_K_MT.. in
+        place (device tensors)."""
+        from . import fields
+        T = self.torch
+        n = conc.shape[1]
+        _want(hetin, "hetin", (fields.NHIN, n), T.float64)
+        _want(conc, "conc", (self.mech.nspec, n), T.float64)
+        _want(met, "met", (fields.NMET, n), T.float64)
+        _check(self.L.gckpp_sulfur_aq_dev(self.h, n, _ptr(hetin), _ptr(conc), _ptr(met), _stream()),
+               "gckpp_sulfur_aq_dev")
+        return met
 
     def aout(self, conc, rconst):
         """Fun's reaction rates A[NREACT][ncell]."""

@@ -358,3 +358,66 @@ def rich_het(rng, ncell):
     h[F["fracOrgAer"][0]] = u(0.0, 1.0)
     h[F["fracInorgAer"][0]] = u(0.0, 1.0)
     return h
+
+
+def model_hetin(rng, ncell):
+    """The model's per-cell fields that fullchem_SetStateHet reads
+    (hetin[NHIN][ncell], fields.HIN): clouds (none, liquid, ice, mixed),
+    temperatures across the ice-radius regimes (202 K, 217 K), land and
+    ocean boxes, aerosol of every type, cloud and aerosol pH (some <= 0),
+    sea-salt alkalinity with empty boxes and ratios above one, PSC states
+    in and out of the stratosphere."""
+    F = fields.HIN
+    h = np.zeros((fields.NHIN, ncell))
+    u = lambda lo, hi: rng.uniform(lo, hi, ncell)
+    lu = lambda lo, hi: 10.0 ** rng.uniform(np.log10(lo), np.log10(hi), ncell)
+    some = lambda p: rng.uniform(0, 1, ncell) < p
+    put = lambda name, v, k=0: h.__setitem__(F[name][0] + k, v)
+    cldf = np.where(some(0.3), 0.0, u(-0.02, 1.02))
+    put("CLDF", cldf)
+    put("QI", np.where(some(0.4), 0.0, lu(1e-8, 1e-4)))
+    put("QL", np.where(some(0.3), 0.0, lu(1e-8, 1e-3)))
+    airvol = lu(1e8, 1e12)
+    airden = lu(0.01, 1.3)
+    put("AIRVOL", airvol)
+    put("AIRDEN", airden)
+    put("AD", airden * airvol)
+    land = u(0, 1)
+    put("FRLAND", land)
+    put("FROCEAN", (1.0 - land) * u(0, 1))
+    put("T", u(180.0, 310.0))
+    put("aClArea", np.where(some(0.1), 0.0, lu(1e-9, 1e-5)))
+    put("aClRadi", lu(1e-6, 1e-4))
+    for k in range(2):
+        put("IsorropAeroH2O", lu(1e-12, 1e-8), k)
+        put("IsorropAeropH", u(-1.0, 8.0), k)
+    for k in range(14):
+        put("AeroArea", np.where(some(0.05), 0.0, lu(1e-9, 1e-5)), k)
+        put("AeroRadi", lu(1e-6, 1e-3), k)
+        put("WetAeroArea", lu(1e-9, 1e-5), k)
+        put("AeroH2O", lu(1e-10, 1e-4), k)
+    # no sea-salt water without sea-salt area (the NO3 uptake on sea salt
+    # divides the water by the aerosol volume)
+    h[F["IsorropAeroH2O"][0]][h[F["aClArea"][0]] == 0] = 0.0
+    h[F["IsorropAeroH2O"][0] + 1][h[F["AeroArea"][0] + 11] == 0] = 0.0
+    put("OMOC_POA", u(1.4, 2.1))
+    put("OMOC_OPOA", u(1.4, 2.1))
+    put("HSO3_aq", np.where(some(0.3), 0.0, lu(1e-9, 1e-4)))
+    put("SO3_aq", np.where(some(0.3), 0.0, lu(1e-12, 1e-6)))
+    put("IsorropBisulfate", u(0.0, 5.0))
+    put("IsorropNitrate", u(0.0, 5.0))
+    put("IsorropSulfate", u(0.0, 5.0))
+    put("IsorropHplus", lu(1e-8, 1e-1))
+    put("pHCloud", np.where(some(0.1), u(-1.0, 0.0), u(2.0, 7.5)))
+    sala, salc = lu(1e-12, 1e-8), lu(1e-12, 1e-8)
+    put("SALA", np.where(some(0.1), 0.0, sala))
+    put("SALAAL", sala * u(0.0, 1.2))
+    put("SALC", np.where(some(0.1), 0.0, salc))
+    put("SALCAL", salc * u(0.0, 1.2))
+    put("fupdateHOBr", u(0.0, 1.0))
+    put("fupdateHOCl", u(0.0, 1.0))
+    for k in range(11):
+        put("KHETI_SLA", u(0.0, 0.3), k)
+    put("InStratosphere", some(0.3).astype(float))
+    put("STATE_PSC", rng.integers(0, 4, ncell).astype(float))
+    return h

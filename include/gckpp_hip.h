@@ -149,6 +149,53 @@ int gckpp_fullchem_step(gckpp_ctx *ctx, int64_t ncell, double dt,
                         int64_t het_stride, const double *hstart, double *conc,
                         double *hnew, int32_t *istatus, int32_t *ierr);
 
+/* The model side of Do_FullChem on the device.  gckpp_set_species_map
+ * (host arrays, once): map[NSPEC] the model species (0-based, of nmodel)
+ * of each KPP species or -1 (State_Chm%Map_KppSpc, fullchem_mod.F90:
+ * 817-821), mw_g[nmodel] molecular weights [g/mol], zero[NSPEC] (or null)
+ * 1 for the prod/loss dummies that start each step at zero (:942-946),
+ * alk_equiv 1 (fullchem, the default configuration: sulfur sea-salt
+ * chemistry in KPP) to convert the alkalinity tracers SALAAL and SALCAL to
+ * equivalents after the gather and back before the scatter
+ * (fullchem_ConvertAlkToEquiv / fullchem_ConvertEquivToAlk,
+ * KPP/fullchem/fullchem_SulfurChemFuncs.F90:67-105, called at
+ * fullchem_mod.F90:2034-2038 and :1285-1287: C * (MW * 7e-5)).
+ * gckpp_gather_dev: species[nmodel][ncell] in kg/kg dry air and
+ * airden[ncell] [kg/m3] -> conc[NSPEC][ncell] in molec/cm3, the unit
+ * conversion of Convert_Spc_Units (GeosUtil/unitconv_mod.F90:1287-1391,
+ * Conc * AIRDEN * (AVO/MW_kg)/1e6) fused with the gather; unmapped KPP
+ * species 0.  gckpp_scatter_dev: the mapped species back, MAX(C,0)
+ * (:1329-1350) converted to kg/kg dry (:1407-1515, C * 1e6/(AVO/MW_kg) /
+ * AIRDEN); the other model species untouched. */
+int gckpp_set_species_map(gckpp_ctx *ctx, int32_t nmodel, const int32_t *map,
+                          const double *mw_g, const int32_t *zero, int32_t alk_equiv);
+int gckpp_gather_dev(gckpp_ctx *ctx, int64_t ncell, const double *species,
+                     const double *airden, double *conc, void *stream);
+int gckpp_scatter_dev(gckpp_ctx *ctx, int64_t ncell, const double *conc,
+                      const double *airden, double *species, void *stream);
+
+/* fullchem_SetStateHet (KPP/fullchem/fullchem_HetStateFuncs.F90:48-244, called
+ * per cell at GeosCore/fullchem_mod.F90:877-888) for a batch: the HetState
+ * record het[NHET][ncell] of the fullchem rate laws from the model's fields
+ * hetin[NHIN][ncell] (State_Met/State_Chm values, layout HIN_* of
+ * cell_fields.h) and the cell's KPP concentrations conc[NSPEC][ncell]
+ * (molec/cm3, after the gather), with Input_Opt's nAeroType (0..14),
+ * gamma_HO2, LPSCCHEM and TurnOffHetRates.  Includes Cld_Params
+ * (rateLawUtilFuncs.F90:281-417), Get_Theta_Ice (:259-301) and Halide_Conc
+ * (:316-616).  fullchem contexts only. */
+int gckpp_set_state_het_dev(gckpp_ctx *ctx, int64_t ncell, const double *hetin,
+                            const double *conc, int32_t naerotype, double gamma_ho2,
+                            int32_t lpscchem, int32_t turn_off_het, double *het, void *stream);
+/* fullchem_SulfurAqChem (KPP/fullchem/fullchem_SulfurChemFuncs.F90:120-305,
+ * via Set_Sulfur_Chem_Rates, GeosCore/fullchem_mod.F90:1964-2074) for a
+ * batch: K_MT(1:6) into the met record's rows MET_K_MT..+5 from hetin's
+ * WetAeroArea and AeroRadi of aerosol types 11 and 12, the KPP vector conc
+ * (alkalinity tracers in equivalents, as gckpp_gather_dev with alk_equiv
+ * leaves them) and met's TEMP and NUMDEN.  Molecular weights from
+ * gckpp_set_species_consts.  fullchem contexts only. */
+int gckpp_sulfur_aq_dev(gckpp_ctx *ctx, int64_t ncell, const double *hetin, const double *conc,
+                        double *met, void *stream);
+
 /* Components, for parity checks against the reference routines:
  * Fun_SPLIT (gckpp_Function.F90:2172-5060): vdot[NVAR][ncell]
  * Jac_SP    (gckpp_Jacobian.F90:48-20887):  jvs[LU_NONZERO][ncell]

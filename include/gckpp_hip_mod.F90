@@ -14,6 +14,8 @@ MODULE gckpp_hip_mod
   PUBLIC :: gckpp_mech_sizes, gckpp_rates, gckpp_fullchem_step, gckpp_set_persistent_handoff
   PUBLIC :: gckpp_step_failures, gckpp_aout, gckpp_kppsa_write, gckpp_kppsa_read
   PUBLIC :: gckpp_kpph_restart_write, gckpp_kpph_restart_read
+  PUBLIC :: gckpp_set_species_map, gckpp_gather_dev, gckpp_scatter_dev
+  PUBLIC :: gckpp_set_state_het_dev, gckpp_sulfur_aq_dev
   INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_NAME_LEN = 64
 
   ! gckpp_kppsa_meta: metadata and integrator fields of a KPP-Standalone
@@ -185,6 +187,49 @@ MODULE gckpp_hip_mod
       REAL(C_DOUBLE),         INTENT(OUT) :: kpph(*)
       INTEGER(C_INT32_T),     INTENT(OUT) :: found
       CHARACTER(KIND=C_CHAR), INTENT(OUT) :: model_tag(*)
+    END FUNCTION
+    ! Map_KppSpc as 0-based model species (-1: none), molecular weights
+    ! [g/mol] of the nmodel model species, prod/loss dummies (1) or null,
+    ! alk_equiv 1: SALAAL/SALCAL to equivalents and back (fullchem)
+    INTEGER(C_INT) FUNCTION gckpp_set_species_map(ctx, nmodel, map, mw_g, zero, alk_equiv) &
+         BIND(C, name="gckpp_set_species_map")
+      IMPORT :: C_INT, C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR),        VALUE      :: ctx
+      INTEGER(C_INT32_T), VALUE      :: nmodel, alk_equiv
+      INTEGER(C_INT32_T), INTENT(IN) :: map(*)
+      REAL(C_DOUBLE),     INTENT(IN) :: mw_g(*)
+      TYPE(C_PTR),        VALUE      :: zero
+    END FUNCTION
+    ! device pointers: SPECIES(ncell,nmodel) kg/kg dry, AIRDEN(ncell) kg/m3,
+    ! C(ncell,NSPEC) molec/cm3 (gather: C out; scatter: SPECIES out)
+    INTEGER(C_INT) FUNCTION gckpp_gather_dev(ctx, ncell, species, airden, conc, stream) &
+         BIND(C, name="gckpp_gather_dev")
+      IMPORT :: C_INT, C_INT64_T, C_PTR
+      TYPE(C_PTR),        VALUE :: ctx, species, airden, conc, stream
+      INTEGER(C_INT64_T), VALUE :: ncell
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION gckpp_scatter_dev(ctx, ncell, conc, airden, species, stream) &
+         BIND(C, name="gckpp_scatter_dev")
+      IMPORT :: C_INT, C_INT64_T, C_PTR
+      TYPE(C_PTR),        VALUE :: ctx, conc, airden, species, stream
+      INTEGER(C_INT64_T), VALUE :: ncell
+    END FUNCTION
+    ! fullchem_SetStateHet for a batch: het(ncell, NHET) from hetin(ncell, NHIN)
+    ! and conc(ncell, NSPEC), device pointers
+    INTEGER(C_INT) FUNCTION gckpp_set_state_het_dev(ctx, ncell, hetin, conc, naerotype, &
+         gamma_ho2, lpscchem, turn_off_het, het, stream) BIND(C, name="gckpp_set_state_het_dev")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR),        VALUE :: ctx, hetin, conc, het, stream
+      INTEGER(C_INT64_T), VALUE :: ncell
+      INTEGER(C_INT32_T), VALUE :: naerotype, lpscchem, turn_off_het
+      REAL(C_DOUBLE),     VALUE :: gamma_ho2
+    END FUNCTION
+    ! fullchem_SulfurAqChem for a batch: K_MT(1:6) into met(ncell, NMET)
+    INTEGER(C_INT) FUNCTION gckpp_sulfur_aq_dev(ctx, ncell, hetin, conc, met, stream) &
+         BIND(C, name="gckpp_sulfur_aq_dev")
+      IMPORT :: C_INT, C_INT64_T, C_PTR
+      TYPE(C_PTR),        VALUE :: ctx, hetin, conc, met, stream
+      INTEGER(C_INT64_T), VALUE :: ncell
     END FUNCTION
     TYPE(C_PTR) FUNCTION gckpp_last_error() BIND(C, name="gckpp_last_error")
       IMPORT :: C_PTR

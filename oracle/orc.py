@@ -140,3 +140,71 @@ class Oracle:
         self.lib.orc_euler(self._p(), n, c.ctypes.data_as(_D), r.ctypes.data_as(_D), tin, tout,
                            ic.ctypes.data_as(_I), ierr.ctypes.data_as(_I))
         return np.ascontiguousarray(c.T), ierr
+
+
+# ---- fullchem_SetStateHet restatement (oracle/setstatehet.c) ----------------
+
+SETHET_SPECIES = ("SO2", "HNO3", "HCl", "HBr", "BrSALA", "BrSALC", "SALACL", "SALCCL",
+                  "NIT", "NITs", "SO4")
+
+
+def _lib():
+    L = ctypes.CDLL(str(lib_path()))
+    i64 = ctypes.c_int64
+    L.orc_set_state_het.argtypes = [i64, _D, _D, _I, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_int, _D]
+    L.orc_cld_params.argtypes = [ctypes.c_double] * 8 + [_D]
+    L.orc_l2g.argtypes = [ctypes.c_double] * 6
+    L.orc_l2g.restype = ctypes.c_double
+    L.orc_safediv.argtypes = [ctypes.c_double] * 3
+    L.orc_safediv.restype = ctypes.c_double
+    L.orc_sulfur_aq.argtypes = [i64, _D, _D, _I, _D, _D]
+    return L
+
+
+def set_state_het(mech, hetin, conc, naerotype=14, gamma_ho2=0.2, lpscchem=True,
+                  turn_off_het=False):
+    """het[NHET][ncell] from hetin[NHIN][ncell] and conc[NSPEC][ncell]."""
+    import sys
+    sys.path.insert(0, str(_HERE.parent / "geos-chem_amd"))
+    from gckpp_amd import fields
+    hetin = np.ascontiguousarray(hetin, dtype=np.float64)
+    conc = np.ascontiguousarray(conc, dtype=np.float64)
+    n = conc.shape[1]
+    assert hetin.shape == (fields.NHIN, n) and conc.shape[0] == mech.nspec
+    sp = _i32([mech.species.index(s) for s in SETHET_SPECIES])
+    out = np.full((fields.NHET, n), np.nan)
+    _lib().orc_set_state_het(n, hetin.ctypes.data_as(_D), conc.ctypes.data_as(_D),
+                             sp.ctypes.data_as(_I), int(naerotype), float(gamma_ho2),
+                             int(bool(lpscchem)), int(bool(turn_off_het)), out.ctypes.data_as(_D))
+    return out
+
+
+def cld_params(ad, cldf, frland, frocean, qi, ql, t, vair):
+    """Cld_Params of one cell: (rLiq, rIce, ALiq, VLiq, AIce, VIce)."""
+    out = (ctypes.c_double * 6)()
+    _lib().orc_cld_params(ad, cldf, frland, frocean, qi, ql, t, vair, out)
+    return np.array(list(out))
+
+
+def l2g(k0, cr, pka, tk, h2oliq, ph):
+    return _lib().orc_l2g(k0, cr, pka, tk, h2oliq, ph)
+
+
+def safediv(num, den, alt):
+    return _lib().orc_safediv(num, den, alt)
+
+
+def sulfur_aq(mech, hetin, conc, met):
+    """fullchem_SulfurAqChem: a copy of met[NMET][ncell] with K_MT(1:6)
+    written (conc's SALAAL, SALCAL in equivalents)."""
+    hetin = np.ascontiguousarray(hetin, dtype=np.float64)
+    conc = np.ascontiguousarray(conc, dtype=np.float64)
+    out = np.array(met, dtype=np.float64, order="C", copy=True)
+    n = conc.shape[1]
+    sp = _i32([mech.species.index(s) for s in ("SO2", "HNO3", "HCl", "O3", "SALAAL", "SALCAL")])
+    mw = np.asarray(mech.species_consts["mw"], dtype=np.float64)
+    srmw = np.sqrt(mw[[mech.species.index(s) for s in ("SO2", "HCl", "HNO3")]])
+    _lib().orc_sulfur_aq(n, hetin.ctypes.data_as(_D), conc.ctypes.data_as(_D),
+                         sp.ctypes.data_as(_I), srmw.ctypes.data_as(_D), out.ctypes.data_as(_D))
+    return out

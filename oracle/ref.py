@@ -127,3 +127,54 @@ class RefKPP:
                                    int(nthreads))
         return c, ist, hnew, ierr
 
+
+
+# ---- pieces of fullchem_SetStateHet's call tree, compiled from the reference
+# (ref_driver.F90 wrappers of rateLawUtilFuncs' Cld_Params and SafeDiv and of
+# henry_mod's CALC_KH/CALC_HEFF)
+
+def _vec(*xs):
+    n = max(np.size(x) for x in xs)
+    return n, [np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,)))
+               for x in xs]
+
+
+def _flib():
+    L = ctypes.CDLL(str(lib_path("fullchem")))
+    i64 = ctypes.c_int64
+    L.ref_cld_params.argtypes = [i64] + [_D] * 9
+    L.ref_safediv.argtypes = [i64] + [_D] * 4
+    L.ref_l2g.argtypes = [i64] + [_D] * 7
+    L.ref_uptake.argtypes = [i64] + [_D] * 9
+    return L
+
+
+def cld_params(ad, cldf, frland, frocean, qi, ql, t, vair):
+    """[6][n]: rLiq, rIce, ALiq, VLiq, AIce, VIce per element."""
+    n, a = _vec(ad, cldf, frland, frocean, qi, ql, t, vair)
+    out = np.zeros((6, n))
+    _flib().ref_cld_params(n, *[_d(x) for x in a], _d(out))
+    return out
+
+
+def safediv(num, den, alt):
+    n, a = _vec(num, den, alt)
+    out = np.zeros(n)
+    _flib().ref_safediv(n, *[_d(x) for x in a], _d(out))
+    return out
+
+
+def l2g(k0, cr, pka, tk, h2oliq, ph):
+    n, a = _vec(k0, cr, pka, tk, h2oliq, ph)
+    out = np.zeros(n)
+    _flib().ref_l2g(n, *[_d(x) for x in a], _d(out))
+    return out
+
+
+def uptake(area, radius, gamma, srmw, numden, temp, cgas, ceduct):
+    """[2][n]: Ars_L1k(area, radius, gamma, srMw) and kIIR1Ltd(cgas, ceduct,
+    that rate) with NUMDEN, SR_TEMP of each element."""
+    n, a = _vec(area, radius, gamma, srmw, numden, temp, cgas, ceduct)
+    out = np.zeros((2, n))
+    _flib().ref_uptake(n, *[_d(x) for x in a], _d(out))
+    return out

@@ -274,6 +274,76 @@ CONTAINS
     !$OMP END PARALLEL DO
   END SUBROUTINE ref_fullchem_step
 
+#if !defined(MECH_HG) && !defined(MECH_CARBON)
+  ! Cld_Params per cell (rateLawUtilFuncs.F90:281-417), vAir [cm3] given:
+  ! out(:,1..6) = rLiq, rIce, ALiq, VLiq, AIce, VIce.
+  SUBROUTINE ref_cld_params( n, ad, cldf, frland, frocean, qi, ql, t, vair, out ) &
+       BIND(C, name="ref_cld_params")
+    USE rateLawUtilFuncs, ONLY : Cld_Params
+    INTEGER(C_INT64_T), VALUE      :: n
+    REAL(C_DOUBLE),     INTENT(IN) :: ad(n), cldf(n), frland(n), frocean(n), qi(n), ql(n)
+    REAL(C_DOUBLE),     INTENT(IN) :: t(n), vair(n)
+    REAL(C_DOUBLE),     INTENT(OUT):: out(n,6)
+    TYPE(HetState) :: Hs
+    INTEGER(C_INT64_T) :: i
+    DO i = 1, n
+       Hs%vAir = vair(i)
+       CALL Cld_Params( ad(i), cldf(i), frland(i), frocean(i), qi(i), ql(i), t(i), Hs )
+       out(i,:) = (/ Hs%rLiq, Hs%rIce, Hs%ALiq, Hs%VLiq, Hs%AIce, Hs%VIce /)
+    ENDDO
+  END SUBROUTINE ref_cld_params
+
+  ! Ars_L1k and kIIR1Ltd per element (rateLawUtilFuncs.F90:77-150), with the
+  ! globals NUMDEN and SR_TEMP of each element's cell:
+  ! out(:,1) = Ars_L1k(area, radius, gamma, srMw), out(:,2) =
+  ! kIIR1Ltd(cgas, ceduct, out(:,1)).
+  SUBROUTINE ref_uptake( n, area, radius, gamma, srmw, numden_in, temp_in, cgas, ceduct, out ) &
+       BIND(C, name="ref_uptake")
+    USE rateLawUtilFuncs, ONLY : Ars_L1k, kIIR1Ltd
+    INTEGER(C_INT64_T), VALUE      :: n
+    REAL(C_DOUBLE),     INTENT(IN) :: area(n), radius(n), gamma(n), srmw(n), numden_in(n)
+    REAL(C_DOUBLE),     INTENT(IN) :: temp_in(n), cgas(n), ceduct(n)
+    REAL(C_DOUBLE),     INTENT(OUT):: out(n,2)
+    INTEGER(C_INT64_T) :: i
+    DO i = 1, n
+       NUMDEN  = numden_in(i)
+       TEMP    = temp_in(i)
+       SR_TEMP = SQRT( TEMP )
+       out(i,1) = Ars_L1k( area(i), radius(i), gamma(i), srmw(i) )
+       out(i,2) = kIIR1Ltd( cgas(i), ceduct(i), out(i,1) )
+    ENDDO
+  END SUBROUTINE ref_uptake
+
+  ! SafeDiv per element (rateLawUtilFuncs.F90:459-478).
+  SUBROUTINE ref_safediv( n, num, den, alt, out ) BIND(C, name="ref_safediv")
+    USE rateLawUtilFuncs, ONLY : SafeDiv
+    INTEGER(C_INT64_T), VALUE      :: n
+    REAL(C_DOUBLE),     INTENT(IN) :: num(n), den(n), alt(n)
+    REAL(C_DOUBLE),     INTENT(OUT):: out(n)
+    INTEGER(C_INT64_T) :: i
+    DO i = 1, n
+       out(i) = SafeDiv( num(i), den(i), alt(i) )
+    ENDDO
+  END SUBROUTINE ref_safediv
+
+  ! The liquid/gas ratio of Compute_L2G_Local (fullchem_HetStateFuncs.F90:
+  ! 631-693) from henry_mod's CALC_KH and CALC_HEFF (GeosUtil/henry_mod.F90).
+  SUBROUTINE ref_l2g( n, k0, cr, pka, tk, h2oliq, ph, out ) BIND(C, name="ref_l2g")
+    USE Henry_Mod, ONLY : CALC_KH, CALC_HEFF
+    INTEGER(C_INT64_T), VALUE      :: n
+    REAL(C_DOUBLE),     INTENT(IN) :: k0(n), cr(n), pka(n), tk(n), h2oliq(n), ph(n)
+    REAL(C_DOUBLE),     INTENT(OUT):: out(n)
+    REAL*8  :: KH, HEFF
+    INTEGER :: RC
+    INTEGER(C_INT64_T) :: i
+    DO i = 1, n
+       CALL CALC_KH( k0(i), cr(i), tk(i), KH, RC )
+       CALL CALC_HEFF( pka(i), ph(i), KH, HEFF, RC )
+       out(i) = HEFF * h2oliq(i)
+    ENDDO
+  END SUBROUTINE ref_l2g
+#endif
+
   SUBROUTINE omp_set_num_threads_wrap( n )
     INTEGER(C_INT), INTENT(IN) :: n
 !$  CALL omp_set_num_threads( INT(n) )

@@ -166,10 +166,13 @@ def test_learned_h0_saves_attempts_within_tolerance(fullchem, torch_dev):
     KPPHvalue and never costs attempts on its cold step.  Accuracy: against a
     tight solve (C oracle, RTOL 1e-8) of the cells where the two starts
     differ most and of cells spread over the grid, the learned start is as
-    close as the reference start.  (The repo tolerance bounds each step's
-    error norm, an RMS over species, not each species' global error: a few
-    species -- BrCl, NO at night -- end far from the tight solution whichever
-    start is used, and the two starts differ there by as much.)"""
+    close as the reference start: as many entries beyond the tolerance (+1 %),
+    a 99th percentile at most twice as large, and over the entries above
+    1e5 molec/cm3 a maximum at most twice the reference start's (+5).  (The
+    repo tolerance bounds each step's error norm, an RMS over species, not
+    each species' global error: a few species -- BrCl, O and NO at night --
+    end far from the tight solution whichever start is used, and the two
+    starts differ there by as much.)"""
     print("case: learned h0 gain, 36x23x24 grids, cold and warm steps", flush=True)
     from gckpp_amd import hip, learned, synth
     from gckpp_amd.options import fullchem_icntrl
@@ -246,6 +249,14 @@ def test_learned_h0_saves_attempts_within_tolerance(fullchem, torch_dev):
                  (e_learn > 1).mean()))
         assert (e_learn > 1).mean() <= (e_ref > 1).mean() + 0.01
         assert np.quantile(e_learn, 0.99) <= 2.0 * np.quantile(e_ref, 0.99) + 1.0
+        # the largest errors of either start sit in species that are
+        # numerically zero (atomic O at night); over the entries above
+        # 1e5 molec/cm3 the learned start's worst error is bounded by the
+        # reference start's
+        sig = ct[:nv] > 1e5
+        print("  entries above 1e5 molec/cm3: reference start max %.3g, learned max %.3g"
+              % (e_ref[sig].max(), e_learn[sig].max()))
+        assert e_learn[sig].max() <= 2.0 * e_ref[sig].max() + 5.0
     assert res["warm_learned"][1] < 0.97 * res["warm_ref"][1]
     assert res["cold_learned"][1] <= 1.01 * res["cold_ref"][1]
     ctx.close()
