@@ -81,8 +81,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *p, int bytes
 // divisor).  Every block issues the same loads in the same order, so the
 // waits for them are counted exactly.  Within a level no chunk reads what
 // another chunk of the level stores, so those reads may pass the level's
-// stores; in MODE 1 the next block's operands are requested after the
-// ordering point that ends a level.
+// stores; between levels the programs hold a round of sink headers (MODE 1),
+// whose operands are the only ones requested before the level's last
+// stores.
 template <int MODE, bool DIV, int D, unsigned PADA, class Store>
 __device__ __forceinline__ void run_prog(char *shb, const WProg &p, int lane,
                                          __amdgpu_buffer_rsrc_t rab, __amdgpu_buffer_rsrc_t rcf,
@@ -170,20 +171,14 @@ __device__ __forceinline__ void run_prog(char *shb, const WProg &p, int lane,
       const unsigned f = (fl >> (4 * i)) & 15u;
       const int jn = i + D;  // the block whose operands are requested now, relative to k0
       const unsigned fj = jn < kPF ? (fl >> (4 * jn)) & 15u : (fln >> (4 * (jn - kPF))) & 15u;
-      // MODE 1 (D = 1): the next block's operands wait for a level's end.
-      // The block runs in both branches, so that each has its own order of
-      // LDS operations and the wait for this block's operands in the usual
-      // branch need not drain the next block's (a wait after a join must
-      // hold for both orders).
-      const bool hold = MODE == 1 && (f & F_LEVEL);
-      if (!hold) {
-        fetch(jn % R, rw[jn % kPF][0], rw[jn % kPF][1], fj);
-        run(i % R, f);
-      } else {
-        run(i % R, f);
-        wave_sync();
-        fetch(jn % R, rw[jn % kPF][0], rw[jn % kPF][1], fj);
-      }
+      // The next block's operands are requested before this block's stores
+      // (LDS operations of a wave execute in program order): the programs
+      // put a round of sink headers after every level's last block, so the
+      // block after a level's end reads nothing the level writes.  No
+      // branch here keeps one order of LDS operations, so the wait for a
+      // block's operands leaves the next block's requests in flight.
+      fetch(jn % R, rw[jn % kPF][0], rw[jn % kPF][1], fj);
+      run(i % R, f);
       rw[jn % kPF][0] = tq[(2 * jn) * 64];
       rw[jn % kPF][1] = tq[(2 * jn + 1) * 64];
     }

@@ -36,16 +36,21 @@ Encoding (the solver reads BYTE offsets, element index * 8):
                reaction-scratch and coefficient byte offsets (Vdot,
                Jacobian).  Pad op: (ZERO, ZERO) in LDS phases and (NAB, 0)
                in Vdot/Jacobian -- products -0.0 that leave every sum
-               unchanged.  NB is padded to a multiple of PF (the first
-               padding block starts a round of sink headers: the solver
-               stores a lane's value after every block); PF + 4 pad blocks
-               follow (the solver loads program words PF blocks and
-               operands up to 3 blocks ahead).
+               unchanged.  In the LDS phases a one-block round of sink
+               headers and pad ops follows every level but the last: the
+               solver reads a block's operands while the block before it
+               runs, so the first block of a level must not follow the
+               level's last block directly.  NB is padded to a multiple of
+               PF (the first padding block starts a round of sink headers:
+               the solver stores a lane's value after every block); PF + 4
+               pad blocks follow (the solver loads program words PF blocks
+               and operands up to 3 blocks ahead).
   wv_<P>_flag  4 bits per block (block k in bits 4(k%8)..4(k%8)+3 of word
                k/8): 1 first block of a round, 2 last block of a round,
-               4 the round ends a level (an ordering point follows), 8 some
-               lane of the round divides (LU, backward substitution).
-  WV_<P>_NB    blocks (a multiple of PF), WV_<P>_NR rounds.
+               4 the round ends a level, 8 some lane of the round divides
+               (LU, backward substitution).
+  WV_<P>_NB    blocks (a multiple of PF), WV_<P>_NR rounds (with the
+               levels' sink rounds).
 Phases P: fv (Vdot), jv (Jacobian in LU layout), lu (row-Crout), sf
 (forward substitution), sb (backward substitution).
 """
@@ -220,6 +225,7 @@ def encode(prog, P):
     lds = P not in ("fv", "jv")
     lev_end = set(lev[1:])
     blocks, flags = [], []
+    npad = 0
     padop = (8 * L["ZERO"], 8 * L["ZERO"]) if lds else (8 * pad[0], 8 * pad[1])
 
     def hdr(r, ln):
@@ -257,6 +263,15 @@ def encode(prog, P):
             if bk == nblk - 1 and divides:
                 f |= F_DIV
             flags.append(f)
+        if lds and (r + 1) in lev_end and r + 1 < nr:
+            # the solver requests a block's LDS operands while the block
+            # before it runs, before that block's stores: a round of sink
+            # headers and pad ops after a level's last block keeps the next
+            # level's first operands clear of the level's last results
+            blocks.append([[(8 * (L["SINK"] + ln), 8 * L["ONE"])] + [padop] * (BSTEP - 1)
+                           for ln in range(LANES)])
+            flags.append(F_START | F_END)
+            npad += 1
     nb = len(blocks)
     nb_pad = -(-nb // PF) * PF
     empty = [[padop] * BSTEP for _ in range(LANES)]
@@ -282,7 +297,7 @@ def encode(prog, P):
         for i, f in enumerate(flags[k:k + 8]):
             w |= f << (4 * i)
         flag.append(w)
-    return ops, flag, nb_pad, nr
+    return ops, flag, nb_pad, nr + npad
 
 
 def emit_arrays(prog):
@@ -308,9 +323,8 @@ def emit_arrays(prog):
 def run_phase(prog, P, sh, step, init_from_dest, store, ab=None, coef=None):
     """Executes phase P from its encoded arrays in the solver's order
     (csrc/wave_solver.hip, run_prog): block by block, the operands of a
-    block read D blocks before it runs (D = 3 for Vdot and Jacobian, 1
-    otherwise; in LU and substitutions the reads for the block after a
-    level's last block wait until that block has run), a round's header op
+    block read D blocks before it runs, before the stores of the blocks in
+    between (D = 3 for Vdot and Jacobian, 1 otherwise), a round's header op
     naming the start of its sum, and every lane's value stored after every
     block.  sh: the LDS block (element list, modified in place); ab, coef:
     the reaction scratch (with its +0.0 pad slot) and the coefficient table
@@ -352,9 +366,7 @@ def run_phase(prog, P, sh, step, init_from_dest, store, ab=None, coef=None):
     r = 0
     for k in range(nb):
         f = flag_of(k)
-        hold = lds and (f & F_LEVEL)
-        if not hold:
-            pending[k + D] = fetch(k + D)
+        pending[k + D] = fetch(k + D)
         blk = pending.pop(k)
         for ln in range(LANES):
             hdr, vals = blk[ln]
@@ -371,8 +383,6 @@ def run_phase(prog, P, sh, step, init_from_dest, store, ab=None, coef=None):
                 store(sh, d // 8, x // 8 + 1 if div else 0, acc[ln])
             else:
                 store(sh, d // 8, x, acc[ln])
-        if hold:
-            pending[k + D] = fetch(k + D)
         if f & F_END:
             r += 1
     assert r == nr
