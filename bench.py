@@ -622,6 +622,9 @@ def mixed_bench(args):
     B.set_config(*best)
     reset()
     B.capture(parts_of(), 1200.0)
+    # cells sampled per mechanism (strided) for the first step's accuracy
+    sample = [np.linspace(0, p.ncell - 1, min(p.ncell, 2048)).astype(np.int64) for p in parts]
+    first = None
     nsteps = 24
     # between two chemistry steps the state moves (emissions, transport):
     # synthetic perturbation on the device, outside the timed replays
@@ -638,9 +641,24 @@ def mixed_bench(args):
         torch.cuda.synchronize()
         wall += time.perf_counter() - t0
         attempts.append([float(s["istatus"][2].float().mean().item()) for s in state])
+        if k == 0:   # the first (cold) step's result on the sampled cells, untimed
+            first = [s["conc"][:, torch.from_numpy(ix).cuda()].cpu().numpy()
+                     for s, ix in zip(state, sample)]
         if k % 6 == 5:
             log("replayed %d steps (%.1f s of replays), attempts/cell %s" % (k + 1, wall, attempts[-1]))
     fails = sum(int((s["ierr"] < 0).sum().item()) for s in state)
+    # the first replayed step (cold, Hstart 0) against the compiled reference
+    # on the same cells, per mechanism
+    accuracy, parity = {}, {}
+    if args.cpu_sample > 0:
+        for m, p, ix, c in zip(mechs, parts, sample, first):
+            ic, rc = icntrl_for(m.name)
+            cref = reference_step(m, p.subset(ix), ic, rc, hstart=np.zeros(len(ix)))
+            err = rel_errors(m, c, cref)
+            accuracy["first_step_vs_reference_%s" % m.name] = err
+            parity["C3_first_step_vs_reference_%s" % m.name] = parity_entry(err, len(ix))
+            log("mixed first step vs reference, %s: max scaled error %.3g on %d cells"
+                % (m.name, err["max_scaled_err"], len(ix)))
     out = {
         "metric": "mixed-mechanism cells/sec per 1200 s chem step, 24 graph-replayed steps",
         "value": round(ntot * nsteps / wall, 2), "unit": "cells/s", "n_gpus": 1, "steps": nsteps,
@@ -656,6 +674,8 @@ def mixed_bench(args):
         "failed_cells_last_step": fails,
         "mean_attempts_per_step": {"first": attempts[0], "last": attempts[-1],
                                    "mean": [float(np.mean([a[q] for a in attempts])) for q in range(3)]},
+        "accuracy": accuracy,
+        "parity": parity,
     }
     B.close()
     for c in ctxs:
