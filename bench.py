@@ -777,6 +777,29 @@ def c90_bench(args):
                                          "warm" if warm else "cold", wall))
         return wall, kern_ms, fails, att, outs, ho
 
+    # wave cap of the persistent launches (untimed, on the first face's cold
+    # step): a face has more than twice the lanes the GPU holds at once, and
+    # in that steady state of work stealing fewer resident waves (three per
+    # SIMD instead of every one that fits) can stream the slots faster
+    cap_ms = {}
+    if args.max_waves == 0 and model is not None:
+        ncu = torch.cuda.get_device_properties(0).multi_processor_count
+        f = faces[0]
+        ctx.set_solver(hip.SOLVER_PERSISTENT)
+        for cap in (0, 12 * ncu):
+            ctx.set_persistent_waves(cap)
+            conc.copy_(f["conc0"])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            model.predict(f["met"], f["photol"], conc, safety=args.safety, h0=h0, cls=cls)
+            ctx.set_cell_order(hip.order_by_class(cls))
+            ctx.fullchem_step(f["met"], f["photol"], f["het"], conc, h0, 1200.0, ic, rc)
+            torch.cuda.synchronize()
+            cap_ms[cap] = (time.perf_counter() - t0) * 1e3
+            ctx.set_cell_order(None)
+        best_cap = min(cap_ms, key=cap_ms.get)
+        ctx.set_persistent_waves(best_cap)
+        log("c90 wave cap: %s ms -> %d" % ({k: round(v, 1) for k, v in cap_ms.items()}, best_cap))
     res = {}
     for warm in (False, True):
         if warm:
@@ -853,6 +876,7 @@ def c90_bench(args):
                                    "learned h0 + persistent against plain GPU",
                        "learned_model": model is not None,
                        "persistent_config": ctx.persistent_config(),
+                       "wave_cap_first_face_cold_ms": {str(k): round(v, 1) for k, v in cap_ms.items()},
                        "handoff_max_cells": args.handoff},
             "handoff_cold": None if args.handoff <= 0 else {
                 "cells_handed_off": ho[0], "cell_per_wave_attempts": ho[1],
