@@ -620,6 +620,21 @@ def mixed_bench(args):
         log("mixed config v%d wpb%d: %.1f ms" % (v, g, tune[(v, g)] * 1e3))
     best = min(tune, key=tune.get)
     B.set_config(*best)
+    # wave cap (as on the C90 faces: far more cells than resident lanes):
+    # one cold step with and without three waves per SIMD, untimed
+    cap_ms = {}
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    for cap in ((args.max_waves,) if args.max_waves > 0 else (0, 12 * ncu)):
+        B.set_waves(cap)
+        reset()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        B.step(parts_of(), 1200.0)
+        torch.cuda.synchronize()
+        cap_ms[cap] = (time.perf_counter() - t) * 1e3
+        log("mixed wave cap %d: %.1f ms" % (cap, cap_ms[cap]))
+    best_cap = min(cap_ms, key=cap_ms.get)
+    B.set_waves(best_cap)
     reset()
     B.capture(parts_of(), 1200.0)
     # cells sampled per mechanism (strided) for the first step's accuracy
@@ -668,7 +683,9 @@ def mixed_bench(args):
                                "random 70/20/10, one persistent launch per step, 24 steps from one "
                                "HIP graph, warm start from RSTATUS(3); between steps (untimed) a "
                                "synthetic emission/transport perturbation of the state" % [p.ncell for p in parts],
-                   "persistent_config": {"variant": best[0], "waves_per_group": best[1]},
+                   "persistent_config": {"variant": best[0], "waves_per_group": best[1],
+                                         "wave_cap": best_cap},
+                   "wave_cap_cold_step_ms": {str(k): round(v, 1) for k, v in cap_ms.items()},
                    "handoff_max_cells": args.handoff,
                    "autotune_first_step_ms": {"v%d_wpb%d" % k: round(v * 1e3, 1) for k, v in tune.items()}},
         "failed_cells_last_step": fails,

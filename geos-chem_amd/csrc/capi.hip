@@ -1261,6 +1261,7 @@ struct gckpp_batch {
   int64_t nwaves, slot_fields;
   double *work;           // [cap_waves][slot_fields][64]
   int64_t cap_waves;
+  int64_t max_waves;      // gckpp_batch_set_waves (0: every resident wave)
   unsigned int *counter;
   hipStream_t cap_stream; // stream the graph is captured on
   hipGraph_t graph;
@@ -1279,6 +1280,7 @@ static int batch_config(gckpp_batch *b, int variant, int wpb, int64_t cap_waves)
   if (res <= 0) return -1;
   int64_t nw = res / wpb * wpb;
   if (cap_waves > 0 && nw > cap_waves) nw = cap_waves / wpb * wpb;
+  if (b->max_waves > 0 && nw > b->max_waves) nw = b->max_waves / wpb * wpb;
   if (nw <= 0) return -1;
   b->variant = variant;
   b->wpb = wpb;
@@ -1493,6 +1495,18 @@ extern "C" int gckpp_batch_set_config(gckpp_batch *b, int variant, int waves_per
     return fail("gckpp_batch_set_config: bad variant or waves per workgroup");
   if (batch_config(b, variant, waves_per_group, b->cap_waves) != 0)
     return fail("gckpp_batch_set_config: configuration does not fit");
+  return 0;
+}
+
+extern "C" int gckpp_batch_set_waves(gckpp_batch *b, int64_t max_waves) {
+  if (!b) return fail("gckpp_batch_set_waves: null batch");
+  if (max_waves < 0) return fail("gckpp_batch_set_waves: negative wave count");
+  const int64_t keep = b->max_waves;
+  b->max_waves = max_waves;
+  if (batch_config(b, b->variant, b->wpb, b->cap_waves) != 0) {
+    b->max_waves = keep;
+    return fail("gckpp_batch_set_waves: %lld waves do not hold one workgroup", (long long)max_waves);
+  }
   return 0;
 }
 

@@ -34,7 +34,7 @@ SYMBOLS = [
     "gckpp_set_persistent_split",
     "gckpp_set_persistent_handoff",
     "gckpp_set_persistent_waves",
-    "gckpp_batch_set_handoff",
+    "gckpp_batch_set_handoff", "gckpp_batch_set_waves",
     "gckpp_batch_handoff_count",
     "gckpp_persistent_handoff_stats",
     "gckpp_aout_dev", "gckpp_aout", "gckpp_kppsa_write", "gckpp_kppsa_read",
@@ -147,6 +147,7 @@ def load():
     L.gckpp_set_persistent_handoff.argtypes = [vp, i64]
     L.gckpp_set_persistent_waves.argtypes = [vp, i64]
     L.gckpp_batch_set_handoff.argtypes = [vp, i64]
+    L.gckpp_batch_set_waves.argtypes = [vp, i64]
     L.gckpp_batch_handoff_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.gckpp_persistent_handoff_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
                                                  ctypes.POINTER(ctypes.c_float)]
@@ -615,6 +616,10 @@ class Batch:
     def set_config(self, variant, waves_per_group=1):
         _check(self.L.gckpp_batch_set_config(self.h, int(variant), int(waves_per_group)),
                "gckpp_batch_set_config")
+
+    def set_waves(self, max_waves):
+        """Cap on the persistent launch's waves (0: every resident wave)."""
+        _check(self.L.gckpp_batch_set_waves(self.h, int(max_waves)), "gckpp_batch_set_waves")
 
     @staticmethod
     def part(met=None, photol=None, het=None, conc=None, hstart=None, icntrl=None, rcntrl=None,

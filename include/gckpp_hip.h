@@ -394,6 +394,11 @@ typedef struct {
 int gckpp_batch_init(gckpp_ctx *fullchem, gckpp_ctx *hg, gckpp_ctx *carbon,
                      gckpp_batch **out);
 int gckpp_batch_set_config(gckpp_batch *b, int variant, int waves_per_group);
+/* Cap on the waves of the batch's persistent launch, as
+ * gckpp_set_persistent_waves for a context (0: every resident wave); kept
+ * across gckpp_batch_set_config.  A captured graph keeps the launch it was
+ * captured with. */
+int gckpp_batch_set_waves(gckpp_batch *b, int64_t max_waves);
 /* The tail hand-off of gckpp_set_persistent_handoff for the batch's launch
  * (fullchem and Hg cells; carbon cells finish on their lanes), captured
  * with the step; gckpp_batch_handoff_count: cells the last step handed

@@ -224,6 +224,49 @@ def test_mixed_batch_equals_per_mechanism_steps_and_graph_replays(fullchem, hg, 
         c.close()
 
 
+def test_mixed_batch_wave_cap_bitwise(fullchem, hg, carbon, torch_dev):
+    """The batch's launch with a cap of 8 waves (work stealing over many
+    cells per lane) gives the same bits as every resident wave, eager and
+    replayed; a cap smaller than a workgroup is refused."""
+    print("case: mixed batch with a wave cap", flush=True)
+    from gckpp_amd import hip
+    T = torch_dev
+    inputs = _mixed_inputs(fullchem, hg, carbon)
+    ctxs = [hip.Context(m, 2048) for m, _, _, _ in inputs]
+    B = hip.Batch(*ctxs)
+
+    def run(cap, graph):
+        B.set_waves(cap)
+        st = []
+        for m, w, ic, rc in inputs:
+            st.append(dict(met=_t(T, w.met), photol=_t(T, w.photol), het=_t(T, w.het),
+                           conc=_t(T, w.conc), hstart=_t(T, w.hstart),
+                           istatus=T.zeros((20, w.ncell), dtype=T.int32, device="cuda"),
+                           ierr=T.zeros((w.ncell,), dtype=T.int32, device="cuda")))
+        parts = [B.part(icntrl=inp[2], rcntrl=inp[3], **d) for inp, d in zip(inputs, st)]
+        if graph:
+            B.capture(parts, 1200.0)
+            B.replay()
+        else:
+            B.step(parts, 1200.0)
+        T.cuda.synchronize()
+        return [(d["conc"].cpu().numpy(), d["hstart"].cpu().numpy(), d["istatus"].cpu().numpy(),
+                 d["ierr"].cpu().numpy()) for d in st]
+
+    full = run(0, False)
+    for cap, graph in ((8, False), (8, True)):
+        got = run(cap, graph)
+        for a, b in zip(full, got):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+    B.set_config(4, 4)
+    with pytest.raises(hip.GckppError, match="waves"):
+        B.set_waves(2)
+    B.close()
+    for c in ctxs:
+        c.close()
+
+
 @pytest.mark.parametrize("variant,wpb", [(7, 1), (8, 1), (8, 4), (9, 2)])
 @pytest.mark.parametrize("name,forced", [("fullchem", False), ("fullchem", True), ("Hg", False),
                                          ("carbon", False)])
