@@ -59,7 +59,7 @@ def parse():
     p.add_argument("--split", type=int, default=4096,
                    help="warm step: cells (those of the smallest predicted h0) run on the "
                         "cell-per-wave kernel beside the lane kernel, per launch (0: one kernel)")
-    p.add_argument("--handoff", type=int, default=32768,
+    p.add_argument("--handoff", type=int, default=98304,
                    help="persistent solver: once at most this many cells of a lane-kernel launch "
                         "are unfinished, they move to the cell-per-wave kernel (0: off)")
     p.add_argument("--no-c90", action="store_true",
