@@ -925,7 +925,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     out = {"grid": grid_bench, "mixed": mixed_bench, "c90": c90_bench}[args.workload](args)
     if args.workload == "grid" and not args.no_c90:
-        # the C90-sized step (the north-star workload) beside the grid's line:
+        # the C90-sized step (3,499,200 cells in six chunks) beside the grid's line:
         # one cold and one warm step of each leg, with its own roofline,
         # accuracy, parity and CPU baseline
         torch.cuda.empty_cache()
