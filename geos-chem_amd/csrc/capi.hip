@@ -1011,6 +1011,63 @@ extern "C" int gckpp_sulfur_aq_dev(gckpp_ctx *c, int64_t ncell, const double *he
   return 0;
 }
 
+// Host-pointer producers, for a host that gathers the model's fields into
+// host arrays (any ncell, in chunks of at most max_cells staged through the
+// device, synchronous): the model-field record hetin rides in the staging
+// rows of PHOTOL (NHIN <= NPHOTOL).
+static_assert(NHIN <= NPHOTOL, "hetin staging");
+static int set_het_args(gckpp_ctx *c, int64_t ncell, int32_t naerotype, const char *who) {
+  if (!c) return fail("%s: null context", who);
+  if (c->mech != MECH_FULLCHEM)
+    return fail("%s: the HetState record is fullchem's (context mechanism %d)", who, c->mech);
+  if (ncell < 0) return fail("%s: negative ncell", who);
+  if (naerotype < 0 || naerotype > 14) return fail("%s: nAeroType %d outside 0..14", who, naerotype);
+  return 0;
+}
+
+extern "C" int gckpp_set_state_het(gckpp_ctx *c, int64_t ncell, const double *hetin,
+                                   const double *conc, int32_t naerotype, double gamma_ho2,
+                                   int32_t lpscchem, int32_t turn_off_het, double *het) {
+  if (set_het_args(c, ncell, naerotype, "gckpp_set_state_het")) return -1;
+  if (ncell == 0) return 0;
+  if (!hetin || !conc || !het) return fail("gckpp_set_state_het: null argument");
+  if (host_stage(c, "gckpp_set_state_het")) return -1;
+  for (int64_t c0 = 0; c0 < ncell; c0 += c->wstride) {
+    const int64_t n = ncell - c0 < c->wstride ? ncell - c0 : c->wstride;
+    const Stage v = stage_views(c, n);
+    HIP_OK(stage_in(v.photol, hetin, NHIN, n, ncell, c0));
+    HIP_OK(stage_in(v.conc, conc, c->info.nspec, n, ncell, c0));
+    launch_set_state_het(n, v.photol, v.conc, v.het, naerotype, gamma_ho2, lpscchem, turn_off_het,
+                         nullptr);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(stage_out(het, v.het, NHET, n, ncell, c0));
+  }
+  return 0;
+}
+
+extern "C" int gckpp_sulfur_aq(gckpp_ctx *c, int64_t ncell, const double *hetin,
+                               const double *conc, double *met) {
+  if (set_het_args(c, ncell, 0, "gckpp_sulfur_aq")) return -1;
+  if (ncell == 0) return 0;
+  if (!hetin || !conc || !met) return fail("gckpp_sulfur_aq: null argument");
+  if (host_stage(c, "gckpp_sulfur_aq")) return -1;
+  for (int64_t c0 = 0; c0 < ncell; c0 += c->wstride) {
+    const int64_t n = ncell - c0 < c->wstride ? ncell - c0 : c->wstride;
+    const Stage v = stage_views(c, n);
+    HIP_OK(stage_in(v.photol, hetin, NHIN, n, ncell, c0));
+    HIP_OK(stage_in(v.conc, conc, c->info.nspec, n, ncell, c0));
+    HIP_OK(stage_in(v.met, met, NMET, n, ncell, c0));
+    launch_sulfur_aq(n, v.photol, v.conc, c->consts, v.met, nullptr);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    // only the K_MT rows change
+    HIP_OK(stage_out(met + (int64_t)MET_K_MT * ncell, v.met + (int64_t)MET_K_MT * n, 6, n, ncell,
+                     c0));
+  }
+  return 0;
+}
+
 // Host-pointer Aout (Fun's reaction rates): per chunk, conc and rconst in,
 // the rates computed in place of the staged rconst, copied out.
 extern "C" int gckpp_aout(gckpp_ctx *c, int64_t ncell, const double *conc, const double *rconst,

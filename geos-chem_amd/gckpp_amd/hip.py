@@ -40,7 +40,7 @@ SYMBOLS = [
     "gckpp_aout_dev", "gckpp_aout", "gckpp_kppsa_write", "gckpp_kppsa_read",
     "gckpp_kpph_restart_write", "gckpp_kpph_restart_read", "gckpp_integrate_noalloc_dev",
     "gckpp_set_species_map", "gckpp_gather_dev", "gckpp_scatter_dev", "gckpp_set_state_het_dev",
-    "gckpp_sulfur_aq_dev",
+    "gckpp_sulfur_aq_dev", "gckpp_set_state_het", "gckpp_sulfur_aq",
     "gckpp_last_error",
 ]
 MLP_NPARAM = 41360
@@ -118,6 +118,9 @@ def load():
     L.gckpp_gather_dev.argtypes = [vp, i64, dp, dp, dp, vp]
     L.gckpp_scatter_dev.argtypes = [vp, i64, dp, dp, dp, vp]
     L.gckpp_sulfur_aq_dev.argtypes = [vp, i64, dp, dp, dp, vp]
+    L.gckpp_set_state_het.argtypes = [vp, i64, dp, dp, ctypes.c_int32, ctypes.c_double,
+                                      ctypes.c_int32, ctypes.c_int32, dp]
+    L.gckpp_sulfur_aq.argtypes = [vp, i64, dp, dp, dp]
     L.gckpp_set_state_het_dev.argtypes = [vp, i64, dp, dp, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_int32, ctypes.c_int32, dp, vp]
     L.gckpp_kppsa_write.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(KppsaMeta),
@@ -589,6 +592,36 @@ class Context:
         _check(self.L.gckpp_sulfur_aq_dev(self.h, n, _ptr(hetin), _ptr(conc), _ptr(met), _stream()),
                "gckpp_sulfur_aq_dev")
         return met
+
+    def set_state_het_host(self, hetin, conc, naerotype=14, gamma_ho2=0.2, lpscchem=True,
+                           turn_off_het=False):
+        """The host-pointer fullchem_SetStateHet: numpy hetin[NHIN][n],
+        conc[NSPEC][n] -> het[NHET][n]."""
+        from . import fields
+        hetin = np.ascontiguousarray(hetin, dtype=np.float64)
+        conc = np.ascontiguousarray(conc, dtype=np.float64)
+        n = conc.shape[1]
+        if hetin.shape != (fields.NHIN, n) or conc.shape[0] != self.mech.nspec:
+            raise GckppError("set_state_het_host: shapes %s, %s" % (hetin.shape, conc.shape))
+        het = np.empty((fields.NHET, n))
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        _check(self.L.gckpp_set_state_het(self.h, n, P(hetin), P(conc), int(naerotype),
+                                          float(gamma_ho2), int(bool(lpscchem)),
+                                          int(bool(turn_off_het)), P(het)), "gckpp_set_state_het")
+        return het
+
+    def sulfur_aq_host(self, hetin, conc, met):
+        """The host-pointer fullchem_SulfurAqChem: K_MT into a copy of met."""
+        from . import fields
+        hetin = np.ascontiguousarray(hetin, dtype=np.float64)
+        conc = np.ascontiguousarray(conc, dtype=np.float64)
+        out = np.array(met, dtype=np.float64, order="C", copy=True)
+        n = conc.shape[1]
+        if hetin.shape != (fields.NHIN, n) or out.shape != (fields.NMET, n):
+            raise GckppError("sulfur_aq_host: shapes %s, %s" % (hetin.shape, out.shape))
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        _check(self.L.gckpp_sulfur_aq(self.h, n, P(hetin), P(conc), P(out)), "gckpp_sulfur_aq")
+        return out
 
     def aout(self, conc, rconst):
         """Fun's reaction rates A[NREACT][ncell]."""

@@ -195,6 +195,15 @@ int gckpp_set_state_het_dev(gckpp_ctx *ctx, int64_t ncell, const double *hetin,
  * gckpp_set_species_consts.  fullchem contexts only. */
 int gckpp_sulfur_aq_dev(gckpp_ctx *ctx, int64_t ncell, const double *hetin, const double *conc,
                         double *met, void *stream);
+/* The same two with host arrays (Fortran hetin(ncell,NHIN), conc(ncell,NSPEC),
+ * het(ncell,NHET), met(ncell,NMET) as they are), synchronous, any ncell in
+ * chunks of at most max_cells staged through the device; gckpp_sulfur_aq
+ * writes only met's K_MT rows. */
+int gckpp_set_state_het(gckpp_ctx *ctx, int64_t ncell, const double *hetin, const double *conc,
+                        int32_t naerotype, double gamma_ho2, int32_t lpscchem,
+                        int32_t turn_off_het, double *het);
+int gckpp_sulfur_aq(gckpp_ctx *ctx, int64_t ncell, const double *hetin, const double *conc,
+                    double *met);
 
 /* Components, for parity checks against the reference routines:
  * Fun_SPLIT (gckpp_Function.F90:2172-5060): vdot[NVAR][ncell]

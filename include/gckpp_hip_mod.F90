@@ -15,7 +15,7 @@ MODULE gckpp_hip_mod
   PUBLIC :: gckpp_step_failures, gckpp_aout, gckpp_kppsa_write, gckpp_kppsa_read
   PUBLIC :: gckpp_kpph_restart_write, gckpp_kpph_restart_read
   PUBLIC :: gckpp_set_species_map, gckpp_gather_dev, gckpp_scatter_dev
-  PUBLIC :: gckpp_set_state_het_dev, gckpp_sulfur_aq_dev
+  PUBLIC :: gckpp_set_state_het_dev, gckpp_sulfur_aq_dev, gckpp_set_state_het, gckpp_sulfur_aq
   INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_NAME_LEN = 64
 
   ! gckpp_kppsa_meta: metadata and integrator fields of a KPP-Standalone
@@ -230,6 +230,25 @@ MODULE gckpp_hip_mod
       IMPORT :: C_INT, C_INT64_T, C_PTR
       TYPE(C_PTR),        VALUE :: ctx, hetin, conc, met, stream
       INTEGER(C_INT64_T), VALUE :: ncell
+    END FUNCTION
+    ! The same two with host arrays (synchronous, chunked)
+    INTEGER(C_INT) FUNCTION gckpp_set_state_het(ctx, ncell, hetin, conc, naerotype, gamma_ho2, &
+         lpscchem, turn_off_het, het) BIND(C, name="gckpp_set_state_het")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR),        VALUE       :: ctx
+      INTEGER(C_INT64_T), VALUE       :: ncell
+      REAL(C_DOUBLE),     INTENT(IN)  :: hetin(*), conc(*)
+      INTEGER(C_INT32_T), VALUE       :: naerotype, lpscchem, turn_off_het
+      REAL(C_DOUBLE),     VALUE       :: gamma_ho2
+      REAL(C_DOUBLE),     INTENT(OUT) :: het(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION gckpp_sulfur_aq(ctx, ncell, hetin, conc, met) &
+         BIND(C, name="gckpp_sulfur_aq")
+      IMPORT :: C_INT, C_INT64_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR),        VALUE         :: ctx
+      INTEGER(C_INT64_T), VALUE         :: ncell
+      REAL(C_DOUBLE),     INTENT(IN)    :: hetin(*), conc(*)
+      REAL(C_DOUBLE),     INTENT(INOUT) :: met(*)
     END FUNCTION
     TYPE(C_PTR) FUNCTION gckpp_last_error() BIND(C, name="gckpp_last_error")
       IMPORT :: C_PTR

@@ -145,6 +145,32 @@ def test_set_state_het_then_rates_match_reference(fullchem, T):
     assert (err <= 1e-7 * np.abs(want) + 1e-7 * scale).all()
 
 
+def test_host_pointer_producers_equal_device(fullchem, T):
+    """gckpp_set_state_het / gckpp_sulfur_aq on host arrays, 2,500 cells in
+    chunks of 1,000 staged through the device: bitwise the device calls on
+    the whole batch; the host met keeps every row but K_MT."""
+    print("case: host-pointer SetStateHet and SulfurAqChem, chunked", flush=True)
+    from gckpp_amd import fields, hip
+    n = 2500
+    w, hin, conc = _inputs(fullchem, n, 51)
+    _alkalinity(fullchem, conc, 51)
+    dev = lambda a: T.from_numpy(np.ascontiguousarray(a)).cuda()
+    big = hip.Context(fullchem, n)
+    het_d = big.set_state_het(dev(hin), dev(conc), naerotype=12, gamma_ho2=0.1).cpu().numpy()
+    met_d = big.sulfur_aq(dev(hin), dev(conc), dev(w.met)).cpu().numpy()
+    big.close()
+    small = hip.Context(fullchem, 1000)
+    het_h = small.set_state_het_host(hin, conc, naerotype=12, gamma_ho2=0.1)
+    met_h = small.sulfur_aq_host(hin, conc, w.met)
+    small.close()
+    assert np.array_equal(het_h, het_d)
+    assert np.array_equal(met_h, met_d)
+    o = fields.MET["K_MT"][0]
+    rest = np.ones(fields.NMET, dtype=bool)
+    rest[o:o + 6] = False
+    assert np.array_equal(met_h[rest], w.met[rest])
+
+
 def test_set_state_het_rejects_bad_arguments(fullchem, hg, T):
     from gckpp_amd import fields, hip
     n = 64
