@@ -13,7 +13,12 @@
 ! ATOL(NVAR), RTOL(NVAR), MW(NSPEC), HENRY_K0(NSPEC), HENRY_CR(NSPEC),
 ! MET(ncell,NMET), PHOTOL(ncell,NPHOTOL), HET(rows,NHET), C(ncell,NSPEC)
 ! (float64, column-major; ncell = NX*NY*NZ, cells in (I,J,L) order), then
-! the NSPEC species names (64 characters each).
+! the NSPEC species names (64 characters each).  With het rows = -1 the
+! file holds HETIN(ncell,GCKPP_NHIN) -- the model's fields that
+! fullchem_SetStateHet reads -- in place of HET, and the host fills HET and
+! MET's K_MT rows on the GPU (gckpp_set_state_het, gckpp_sulfur_aq: the
+! options nAeroType 14, gamma_HO2 0.2, LPSCCHEM on, het rates on) before
+! the step.
 ! Result file: the step's return code (int32), the cells that failed twice
 ! (int64), C, KPPHvalue (float64), IERR (int32), the nine KppDiags fields
 ! (float32: IntCounts, JacCounts, TotSteps, AccSteps, RejSteps, LuDecomps,
@@ -31,6 +36,8 @@ PROGRAM chem_host
   REAL(C_DOUBLE),     ALLOCATABLE :: atol(:), rtol(:), mw(:), hk0(:), hcr(:), satol(:)
   REAL(C_DOUBLE),     ALLOCATABLE :: met(:,:), phot(:,:), het(:,:), kpph(:), kpph0(:), c(:,:)
   REAL(C_DOUBLE),     ALLOCATABLE :: cinit(:,:), rconst(:,:), rstatus(:,:), aout(:,:)
+  REAL(C_DOUBLE),     ALLOCATABLE :: hetin(:,:)
+  LOGICAL                         :: producers
   INTEGER(C_INT32_T), ALLOCATABLE :: ierr(:), istatus(:,:)
   CHARACTER(LEN=64),  ALLOCATABLE :: names(:)
   CHARACTER(KIND=C_CHAR)          :: tag(GCKPP_NAME_LEN)
@@ -54,11 +61,20 @@ PROGRAM chem_host
   nspec = sz(1); nvar = sz(2); nreact = sz(4); nmet = sz(6); nhet = sz(7); nphot = sz(8)
   ncell = nx * ny * nz
   ALLOCATE(atol(nvar), rtol(nvar), mw(nspec), hk0(nspec), hcr(nspec), names(nspec))
+  producers = hcols < 0
+  IF (producers) THEN
+    hcols = ncell
+    ALLOCATE(hetin(ncell,GCKPP_NHIN))
+  END IF
   ALLOCATE(met(ncell,nmet), phot(ncell,nphot), het(hcols,nhet), c(ncell,nspec), kpph(ncell))
   ALLOCATE(kpph0(ncell))
   ALLOCATE(ierr(ncell), cinit(ncell,nspec), rconst(ncell,nreact), istatus(ncell,20))
   ALLOCATE(rstatus(ncell,20))
-  READ(u) atol, rtol, mw, hk0, hcr, met, phot, het, c, names
+  IF (producers) THEN
+    READ(u) atol, rtol, mw, hk0, hcr, met, phot, hetin, c, names
+  ELSE
+    READ(u) atol, rtol, mw, hk0, hcr, met, phot, het, c, names
+  END IF
   CLOSE(u)
 
   ! Init_FullChem / Init_Mercury: context, tolerances, species constants
@@ -73,6 +89,15 @@ PROGRAM chem_host
   IF (handoff > 0) THEN
     rc = gckpp_set_persistent_handoff(ctx, INT(handoff, C_INT64_T))
     IF (rc /= 0) CALL die('gckpp_set_persistent_handoff')
+  END IF
+
+  ! Set_Sulfur_Chem_Rates' sea-salt part and fullchem_SetStateHet on the GPU
+  IF (producers) THEN
+    rc = gckpp_sulfur_aq(ctx, ncell, hetin, c, met)
+    IF (rc /= 0) CALL die('gckpp_sulfur_aq')
+    rc = gckpp_set_state_het(ctx, ncell, hetin, c, 14_C_INT32_T, 0.2_C_DOUBLE, 1_C_INT32_T, &
+                             0_C_INT32_T, het)
+    IF (rc /= 0) CALL die('gckpp_set_state_het')
   END IF
 
   ! KPPHvalue from the restart file (zero without one, or without the variable)

@@ -17,6 +17,8 @@ MODULE gckpp_hip_mod
   PUBLIC :: gckpp_set_species_map, gckpp_gather_dev, gckpp_scatter_dev
   PUBLIC :: gckpp_set_state_het_dev, gckpp_sulfur_aq_dev, gckpp_set_state_het, gckpp_sulfur_aq
   INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_NAME_LEN = 64
+  ! fields of the model-field record hetin (HIN_* of geos-chem_amd/csrc/cell_fields.h)
+  INTEGER(C_INT), PARAMETER, PUBLIC :: GCKPP_NHIN = 99
 
   ! gckpp_kppsa_meta: metadata and integrator fields of a KPP-Standalone
   ! state file (GeosCore/kppsa_interface_mod.F90:569-843); strings are

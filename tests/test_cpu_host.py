@@ -29,6 +29,13 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) <= set(hip.SYMBOLS) | {"gckpp_step_timings"}
 
 
+def test_fortran_module_record_size_matches_layout():
+    """GCKPP_NHIN of the Fortran module is the hetin record of cell_fields.h."""
+    from gckpp_amd import fields
+    txt = (ROOT / "include" / "gckpp_hip_mod.F90").read_text()
+    assert int(re.search(r"GCKPP_NHIN\s*=\s*(\d+)", txt).group(1)) == fields.NHIN
+
+
 def test_mech_sizes_without_gpu():
     from gckpp_amd import hip, fields
     if not hip.lib_path().exists():

@@ -3,7 +3,9 @@ geos-chem_amd/fortran/chem_host (flang, include/gckpp_hip_mod.F90 and
 geos-chem_amd/fortran/gckpp_chem_mod.F90) reads KPPHvalue from a restart
 file, runs one chemistry step of a gathered 12x10x5 grid with forced
 retries, and writes the cells, the KppDiags fields, the next KPPHvalue as a
-restart file and one active cell's KPP-Standalone state file.
+restart file and one active cell's KPP-Standalone state file; in one case
+the host also fills the HetState record and K_MT from the model's fields
+with the host-pointer producers.
 
 Expected, built here from the device calls of the Python binding:
   first Integrate with Hstart = KPPHvalue; KppDiags = its ISTATUS(1:8) and
@@ -37,17 +39,20 @@ def T():
     return torch
 
 
-def _write_batch(path, m, w, solver, active, max_cells, dt, ic, rc):
+def _write_batch(path, m, w, solver, active, max_cells, dt, ic, rc, hetin=None):
+    """hetin: the model-field record in place of HET (the host then runs the
+    producers)."""
     sc = m.species_consts
     with open(path, "wb") as f:
         f.write(np.array([m.mech_id, solver, 0, active], dtype=np.int32).tobytes())
-        f.write(np.array([NX, NY, NZ, max_cells, w.het.shape[1]], dtype=np.int64).tobytes())
+        f.write(np.array([NX, NY, NZ, max_cells, w.het.shape[1] if hetin is None else -1],
+                         dtype=np.int64).tobytes())
         f.write(np.array([dt], dtype=np.float64).tobytes())
         f.write(np.asarray(ic, dtype=np.int32).tobytes())
         f.write(np.asarray(rc, dtype=np.float64).tobytes())
         for a in (m.atol(), m.rtol(), sc["mw"], sc["henry_k0"], sc["henry_cr"]):
             f.write(np.asarray(a, dtype=np.float64).tobytes())
-        for a in (w.met, w.photol, w.het, w.conc):
+        for a in (w.met, w.photol, w.het if hetin is None else hetin, w.conc):
             f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
         f.write(b"".join(s.encode().ljust(64) for s in m.species))
 
@@ -72,12 +77,17 @@ def _read_result(path, m, n):
     return out
 
 
-@pytest.mark.parametrize("name,solver_name", [("fullchem", "persistent"), ("fullchem", "plain"),
-                                              ("Hg", "persistent")])
-def test_chem_host_protocol_diags_restart_and_state_file(name, solver_name, mech_of, T, tmp_path):
-    print("case: Fortran chem host, %s, %s solver, 600 cells, forced retries" % (name, solver_name),
-          flush=True)
-    from gckpp_amd import hip, options, synth
+@pytest.mark.parametrize("name,solver_name,producers", [
+    ("fullchem", "persistent", False), ("fullchem", "plain", False), ("Hg", "persistent", False),
+    ("fullchem", "persistent", True)])
+def test_chem_host_protocol_diags_restart_and_state_file(name, solver_name, producers, mech_of, T,
+                                                         tmp_path):
+    """(producers: the batch carries the model-field record; the host fills
+    HetState and K_MT with gckpp_set_state_het / gckpp_sulfur_aq, expected
+    here from the device forms.)"""
+    print("case: Fortran chem host, %s, %s solver, 600 cells, forced retries%s"
+          % (name, solver_name, ", producers" if producers else ""), flush=True)
+    from gckpp_amd import fields, hip, options, synth
     assert HOST.exists(), "the Fortran host is built with the library (make -C geos-chem_amd/csrc)"
     m = mech_of(name)
     solver = {"plain": hip.SOLVER_PLAIN, "persistent": hip.SOLVER_PERSISTENT}[solver_name]
@@ -89,6 +99,13 @@ def test_chem_host_protocol_diags_restart_and_state_file(name, solver_name, mech
 
     # cold-start step counts, to set a step limit that fails many first calls
     ctx = hip.Context(m, n)
+    hetin = None
+    if producers:
+        hetin = synth.model_hetin(np.random.default_rng(29), n)
+        hetin[fields.HIN["T"][0]] = w.met[fields.MET["TEMP"][0]]
+        w.met = ctx.sulfur_aq(dev(hetin), dev(w.conc), dev(w.met)).cpu().numpy()
+        w.het = ctx.set_state_het(dev(hetin), dev(w.conc), naerotype=14, gamma_ho2=0.2,
+                                  lpscchem=True, turn_off_het=False).cpu().numpy()
     ctx.set_solver(solver)
     rct = ctx.rates(dev(w.met), dev(w.photol), dev(w.het), dev(w.conc))
     c = dev(w.conc)
@@ -141,7 +158,7 @@ def test_chem_host_protocol_diags_restart_and_state_file(name, solver_name, mech
     rin, rout = tmp_path / "restart_in.nc", tmp_path / "restart_out.nc"
     hip.kpph_restart_write(rin, kpph0.reshape((NX, NY, NZ), order="F"), model_tag="test-model")
     fin, fres, fsa = tmp_path / "batch.bin", tmp_path / "result.bin", tmp_path / "active.txt"
-    _write_batch(fin, m, w, solver, active + 1, 256, dt, ic, rc)
+    _write_batch(fin, m, w, solver, active + 1, 256, dt, ic, rc, hetin=hetin)
     r = subprocess.run([str(HOST), str(fin), str(rin), str(rout), str(fres), str(fsa)],
                        capture_output=True, text=True, timeout=300)
     print(r.stdout[-2000:], r.stderr[-2000:])
