@@ -222,16 +222,39 @@ __device__ void wave_fun(const WaveCell<M> &c, int in0) {
   const double *rct = c.rct();
   double *ab = c.ab();
   wave_sync_global();
+  // kU reactions per lane at a time, every load of the group issued before
+  // the first product (an index past NREACT is clamped to the last
+  // reaction and its product not stored); the factors multiply in the
+  // reactant order up to the list's first -1, as in the lane solvers
+  constexpr int kU = 8;
 #pragma unroll 1
-  for (int r = c.lane; r < M::NREACT; r += 64) {
-    double v = rct[r];
+  for (int r0 = c.lane; r0 < M::NREACT; r0 += 64 * kU) {
+    double v[kU], x[kU][M::RX_MAX];
+    int s[kU][M::RX_MAX];
 #pragma unroll
-    for (int q = 0; q < M::RX_MAX; ++q) {
-      const int s = M::rx_s(r * M::RX_MAX + q);
-      if (s < 0) break;
-      v *= s < M::NVAR ? sh[in0 + s] : sh[M::WV_Y0 + s];
+    for (int u = 0; u < kU; ++u) {
+      const int r = r0 + 64 * u < M::NREACT ? r0 + 64 * u : M::NREACT - 1;
+      v[u] = rct[r];
+#pragma unroll
+      for (int q = 0; q < M::RX_MAX; ++q) s[u][q] = M::rx_s(r * M::RX_MAX + q);
     }
-    ab[r] = v;
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int q = 0; q < M::RX_MAX; ++q) {
+        const int e = s[u][q];
+        x[u][q] = sh[e < 0 ? M::WV_ZERO : (e < M::NVAR ? in0 + e : M::WV_Y0 + e)];
+      }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      bool on = true;
+#pragma unroll
+      for (int q = 0; q < M::RX_MAX; ++q) {
+        on = on && s[u][q] >= 0;
+        if (on) v[u] *= x[u][q];
+      }
+      if (r0 + 64 * u < M::NREACT) ab[r0 + 64 * u] = v[u];
+    }
   }
   wave_sync_global();
   run_prog<0, false, 3, 8u * M::WV_NAB>((char *)sh, M::wprog(WP_FV), c.lane, c.rab(), c.rcf(),
@@ -245,16 +268,35 @@ __device__ int wave_jac(const WaveCell<M> &c, double sign, double diag) {
   const double *rct = c.rct();
   double *ab = c.ab();
   wave_sync_global();
+  // kU partials per lane at a time, loads first (as in wave_fun)
+  constexpr int kU = 8, NQ = M::RX_MAX - 1;
 #pragma unroll 1
-  for (int p = c.lane; p < M::NPART; p += 64) {
-    double v = M::pt_m(p) * rct[M::pt_r(p)];
+  for (int p0 = c.lane; p0 < M::NPART; p0 += 64 * kU) {
+    double m[kU], k[kU], x[kU][NQ > 0 ? NQ : 1];
+    int s[kU][NQ > 0 ? NQ : 1];
 #pragma unroll
-    for (int q = 0; q < M::RX_MAX - 1; ++q) {
-      const int s = M::pt_s(p * (M::RX_MAX - 1) + q);
-      if (s < 0) break;
-      v *= sh[M::WV_Y0 + s];
+    for (int u = 0; u < kU; ++u) {
+      const int p = p0 + 64 * u < M::NPART ? p0 + 64 * u : M::NPART - 1;
+      m[u] = M::pt_m(p);
+      k[u] = rct[M::pt_r(p)];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) s[u][q] = M::pt_s(p * NQ + q);
     }
-    ab[p] = v;
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) x[u][q] = sh[s[u][q] < 0 ? M::WV_ZERO : M::WV_Y0 + s[u][q]];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      double v = m[u] * k[u];
+      bool on = true;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        on = on && s[u][q] >= 0;
+        if (on) v *= x[u][q];
+      }
+      if (p0 + 64 * u < M::NPART) ab[p0 + 64 * u] = v;
+    }
   }
   wave_sync_global();
   run_prog<0, false, 3, 8u * M::WV_NAB>((char *)sh, M::wprog(WP_JV), c.lane, c.rab(), c.rcf(),
@@ -372,35 +414,53 @@ __device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64
   }
   if (singular) { S.ierr = -8; return true; }
   const double H = S.H;
+  // Species loops: species i = lane + 64 k, k < KS, with the loads of all k
+  // issued together (indices past NVAR clamped to the last species, whose
+  // values are not stored).
+  int ic[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) ic[k] = lane + 64 * k < N ? lane + 64 * k : N - 1;
 #pragma unroll 1
   for (int s = 0; s < 4; ++s) {
     wave_sync_global();
+    double v[KS];
     if (s == 0 || !kRosNewF[s]) {
-#pragma unroll 1
-      for (int i = lane; i < N; i += 64) sh[M::WV_X0 + i] = F0[i];
+#pragma unroll
+      for (int k = 0; k < KS; ++k) v[k] = F0[ic[k]];
     } else {
       // the stage point Ynew in X, then Fun(Ynew) into X
+#pragma unroll
+      for (int k = 0; k < KS; ++k) v[k] = sh[M::WV_Y0 + ic[k]];
 #pragma unroll 1
-      for (int i = lane; i < N; i += 64) {
-        double v = sh[M::WV_Y0 + i];
-        for (int j = 0; j < s; ++j) {
-          const double aj = kRosA[s * (s - 1) / 2 + j];
-          if (aj != 0.0) v = fma(aj, c.K(j)[i], v);
+      for (int j = 0; j < s; ++j) {
+        const double aj = kRosA[s * (s - 1) / 2 + j];
+        if (aj != 0.0) {
+          const double *Kj = c.K(j);
+#pragma unroll
+          for (int k = 0; k < KS; ++k) v[k] = fma(aj, Kj[ic[k]], v[k]);
         }
-        sh[M::WV_X0 + i] = v;
       }
+#pragma unroll
+      for (int k = 0; k < KS; ++k)
+        if (lane + 64 * k < N) sh[M::WV_X0 + lane + 64 * k] = v[k];
       wave_sync();
       wave_fun<M>(c, M::WV_X0);
       S.st[0] += 1;
+#pragma unroll
+      for (int k = 0; k < KS; ++k) v[k] = sh[M::WV_X0 + ic[k]];
     }
+#pragma unroll 1
     for (int j = 0; j < s; ++j) {
       const double hc = kRosC[s * (s - 1) / 2 + j] / (dir * H);
       if (hc != 0.0) {
         const double *Kj = c.K(j);
-#pragma unroll 1
-        for (int i = lane; i < N; i += 64) sh[M::WV_X0 + i] = fma(hc, Kj[i], sh[M::WV_X0 + i]);
+#pragma unroll
+        for (int k = 0; k < KS; ++k) v[k] = fma(hc, Kj[ic[k]], v[k]);
       }
     }
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+      if (lane + 64 * k < N) sh[M::WV_X0 + lane + 64 * k] = v[k];
     wave_sync();
     wave_solve<M>(c);
     double *Ks = c.K(s);
@@ -412,22 +472,31 @@ __device__ bool wave_attempt(const IntegrateArgs &a, const WaveCell<M> &c, int64
   // Ynew (kept per lane) and the scaled error terms per species in X, then
   // the error sum in species order (every lane sums the same terms)
   double yn[KS];
+  {
+    double kv[4][KS], y0[KS], at[KS], rt[KS];
 #pragma unroll
-  for (int k = 0; k < KS; ++k) {
-    const int i = lane + 64 * k;
-    yn[k] = 0.0;
-    if (i < N) {
-      const double y0 = sh[M::WV_Y0 + i];
-      double v = y0;
+    for (int k = 0; k < KS; ++k) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) kv[j][k] = c.K(j)[ic[k]];
+      y0[k] = sh[M::WV_Y0 + ic[k]];
+      at[k] = a.atol[P.vectol ? ic[k] : 0];
+      rt[k] = a.rtol[P.vectol ? ic[k] : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int i = lane + 64 * k;
+      double v = y0[k];
+#pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (kRosM[j] != 0.0) v = fma(kRosM[j], c.K(j)[i], v);
+        if (kRosM[j] != 0.0) v = fma(kRosM[j], kv[j][k], v);
       double ye = 0.0;
+#pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (kRosE[j] != 0.0) ye = fma(kRosE[j], c.K(j)[i], ye);
-      const double ymax = max_f(fabs(y0), fabs(v));
-      const double sc = P.vectol ? a.atol[i] + a.rtol[i] * ymax : a.atol[0] + a.rtol[0] * ymax;
-      yn[k] = v;
-      sh[M::WV_X0 + i] = ye / sc;
+        if (kRosE[j] != 0.0) ye = fma(kRosE[j], kv[j][k], ye);
+      const double ymax = max_f(fabs(y0[k]), fabs(v));
+      const double sc = at[k] + rt[k] * ymax;
+      yn[k] = i < N ? v : 0.0;
+      if (i < N) sh[M::WV_X0 + i] = ye / sc;
     }
   }
   wave_sync();
