@@ -35,6 +35,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "fullchem cells/sec per 1200 s chem step at repo RTOL: plain GPU vs learned-h0"
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 peak (vector and matrix)
+PEAK_HBM_TBPS = 8.0          # MI355X HBM3E peak
 # FP64 work per Rodas3 attempt of fullchem, counted from the mechanism tables
 # (first attempt of a step 145.7 kFLOP, a rejected re-attempt 122.3 kFLOP).
 FLOP_ACC, FLOP_REJ = 145.7e3, 122.3e3
@@ -408,7 +409,7 @@ def grid_bench(args):
                                "per attempt from the mechanism tables)",
                      "kernel_ms": round(k_ms, 3),
                      "hbm_TBps": None if traffic is None else round(traffic / (k_ms * 1e-3) / 1e12, 3),
-                     "hbm_frac": None if traffic is None else round(traffic / (k_ms * 1e-3) / 8e12, 4)},
+                     "hbm_frac": None if traffic is None else round(traffic / (k_ms * 1e-3) / 1e12 / PEAK_HBM_TBPS, 4)},
         "solver_stats": {
             "learned": {"mean_attempts": float(nstp.mean()), "mean_rejects": float(L["st"][4].mean()),
                         "failed_cells": int((L["ierr"] < 0).sum()),
@@ -882,6 +883,19 @@ def c90_bench(args):
         flop -= FLOP_ACC * ho[1]
         kms = ho[2]
     achieved = flop / (kms * 1e-3) / 1e12
+    # HBM bytes per chunk launch from the committed rocprof passes over this
+    # workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated, the mean of the
+    # six lane launches), when they measured this configuration
+    traffic = None
+    pc = ctx.persistent_config()
+    tfile = ROOT / "profiles" / "persist_traffic.json"
+    if tfile.exists():
+        for tr in json.loads(tfile.read_text()).get("entries", []):
+            if (tr.get("workload") == "c90" and tr.get("variant") == pc["variant"] and
+                    tr.get("cells") == nface and tr.get("handoff", 0) == args.handoff and
+                    tr.get("waves") == pc["waves"]):
+                traffic = tr.get("bytes_per_launch")
+    hbm = None if traffic is None else 6 * traffic / (kms * 1e-3) / 1e12
     L, P = leg_json(("learned", False)), leg_json(("plain", False))
     Lw, Pw = leg_json(("learned", True)), leg_json(("plain", True))
     return {"metric": METRIC + " (C90-sized)", "value": L["value"], "unit": "cells/s", "n_gpus": 1,
@@ -906,8 +920,11 @@ def c90_bench(args):
                                             "the lane kernel" % args.split
                                             if args.split > 0 else "one persistent kernel")},
             "roofline": {"bound": "fp64_valu", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
-                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6), "traffic": None,
-                         "kernel": "k_persist over the six chunks of the cold step"},
+                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6), "traffic": traffic,
+                         "kernel": "k_persist over the six chunks of the cold step (traffic: per chunk launch)",
+                         "kernel_ms": round(kms, 3),
+                         "hbm_TBps": None if hbm is None else round(hbm, 3),
+                         "hbm_frac": None if hbm is None else round(hbm / PEAK_HBM_TBPS, 4)},
             "accuracy": acc, "parity": parity, "cpu_baseline": base}
 
 
