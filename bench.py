@@ -346,12 +346,15 @@ def grid_bench(args):
     # HBM bytes per launch from the committed rocprof pass (FETCH_SIZE and
     # WRITE_SIZE, copy-calibrated), when it measured this variant on this grid
     traffic = None
+    traffic_src = ("no committed rocprof pass for variant %d, %d cells, hand-off %d"
+                   % (pcfg["variant"], n, args.handoff))
     tfile = ROOT / "profiles" / "persist_traffic.json"
     if tfile.exists():
         for tr in json.loads(tfile.read_text()).get("entries", []):
-            if (tr.get("variant") == pcfg["variant"] and tr.get("cells") == n and
-                    tr.get("handoff", 0) == args.handoff):
+            if (tr.get("workload", "grid") == "grid" and tr.get("variant") == pcfg["variant"] and
+                    tr.get("cells") == n and tr.get("handoff", 0) == args.handoff):
                 traffic = tr.get("bytes_per_launch")
+                traffic_src = tr.get("source", "profiles/k_persist_fullchem_grid72x46x72.md")
 
     base, sidx, refout = cpu_baseline(mech, w, ic, rc, args.cpu_sample)
     accuracy = {"learned_vs_plain_gpu": rel_errors(mech, L["conc"], P["conc"])}
@@ -404,7 +407,7 @@ def grid_bench(args):
         "speedup_vs_plain": round(L["cps"] / P["cps"], 3),
         "roofline": {"bound": "fp64_valu", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
-                     "traffic": traffic,
+                     "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_persist (persistent Rodas3, FP64 sparse; the algorithmic work "
                                "per attempt from the mechanism tables)",
                      "kernel_ms": round(k_ms, 3),
@@ -888,6 +891,8 @@ def c90_bench(args):
     # six lane launches), when they measured this configuration
     traffic = None
     pc = ctx.persistent_config()
+    traffic_src = ("no committed rocprof pass for variant %d, %d waves, hand-off %d"
+                   % (pc["variant"], pc["waves"], args.handoff))
     tfile = ROOT / "profiles" / "persist_traffic.json"
     if tfile.exists():
         for tr in json.loads(tfile.read_text()).get("entries", []):
@@ -895,6 +900,7 @@ def c90_bench(args):
                     tr.get("cells") == nface and tr.get("handoff", 0) == args.handoff and
                     tr.get("waves") == pc["waves"]):
                 traffic = tr.get("bytes_per_launch")
+                traffic_src = tr.get("source")
     hbm = None if traffic is None else 6 * traffic / (kms * 1e-3) / 1e12
     L, P = leg_json(("learned", False)), leg_json(("plain", False))
     Lw, Pw = leg_json(("learned", True)), leg_json(("plain", True))
@@ -921,6 +927,7 @@ def c90_bench(args):
                                             if args.split > 0 else "one persistent kernel")},
             "roofline": {"bound": "fp64_valu", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6), "traffic": traffic,
+                         "traffic_source": traffic_src,
                          "kernel": "k_persist over the six chunks of the cold step (traffic: per chunk launch)",
                          "kernel_ms": round(kms, 3),
                          "hbm_TBps": None if hbm is None else round(hbm, 3),
